@@ -1,0 +1,263 @@
+// mc_device.hpp — device-side data layout and per-event physics of the
+// event-based transport loop (gfx950, wave64).
+//
+// Layout in HBM:
+//   * problem data (surfaces, cells, materials, nuclides, estimators) are the
+//     flat mc_*_desc records of include/mc_api.h, concatenated over the
+//     engine's problems with indices rebased;
+//   * cross-section tables: one energy grid per distinct grid (nuclides with
+//     identical grids share it) plus per-nuclide sigma_s, sigma_c, sigma_t;
+//     each grid carries a bucket table over the IEEE bit pattern of E
+//     (2^kBucketBits buckets per octave) narrowing the bin search;
+//   * the particle bank is structure-of-arrays, one array per field, indexed
+//     by slot; per-slot history tallies are rows of `hist` ([slot][bin]).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mc_physics.hpp"
+
+namespace mc {
+
+constexpr int kBucketBits = 6;                 // buckets per octave = 64
+constexpr int kBucketShift = 52 - kBucketBits;
+
+enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2 };
+
+struct DNuc {            // device nuclide (deduplicated across problems)
+    double A;
+    double xs_s, xs_c, xs_t;
+    int32_t kind, grid, tab, n;   // tab: offset into xs_s/xs_c/xs_t tables
+};
+
+struct DGrid {
+    int32_t e_off, n;             // energies in grid_E[e_off .. e_off+n)
+    int32_t b_off, nb;            // buckets in bucket[b_off .. b_off+nb]
+    int64_t key_base;             // (bits(E_0) >> kBucketShift)
+};
+
+struct DProb {
+    int32_t cell_begin, cell_count;
+    int32_t src_begin, src_count;
+    int32_t bin_begin, bin_count;
+    int32_t param_begin, param_count;
+    uint64_t seed;
+    double wr, ws;
+};
+
+struct Counters {
+    uint32_t nA, nS, nC, pad0;
+    uint64_t next_hist;           // next history key to start
+    uint64_t end_hist;            // one past the last key of this run
+    uint64_t flights;             // Ntrack
+    uint32_t err_lost, err_split;
+};
+
+// Everything a kernel needs, passed by value.
+struct KArgs {
+    // problem data
+    const mc_surface_desc* surf;
+    const mc_cell_desc* cell;
+    const int32_t* cell_surf;
+    const int32_t* cell_sense;
+    const int32_t* nbr_begin;     // per surface: candidate cells after a crossing
+    const int32_t* nbr;
+    const mc_material_desc* mat;
+    const int32_t* mat_nuc;       // device nuclide index per material entry
+    const double* mat_dens;       // N
+    const double* mat_ns;         // per-run scatter density (parameterised materials)
+    const double* mat_nc;         // per-run capture density
+    const int32_t* mat_param;     // global parameter index
+    const mc_param_desc* param;   // material/nuc_slot rebased
+    const double* theta;
+    const DNuc* nuc;
+    const DGrid* grid;
+    const double* grid_E;
+    const int32_t* bucket;
+    const double* tab_s;
+    const double* tab_c;
+    const double* tab_t;
+    const mc_estimator_desc* est; // bin_offset is global
+    const int32_t* att_est;
+    const int32_t* att_idx;
+    const double* egrid;
+    const DProb* prob;
+    const mc_source_desc* src;
+    const double* src_cdf;        // per problem: src_count + 1 entries at cdf_off
+    const int32_t* cdf_off;
+    const uint64_t* jump;         // 64 LCG jump multipliers
+    const uint64_t* run_hb;       // per problem first key of this run (n_prob + 1)
+    int32_t n_prob;
+    int32_t bin_stride;           // hist row length (max bins per problem)
+    int32_t par_stride;           // S/G row length (max params per problem)
+    int32_t pad;
+    uint64_t cap;                 // bank capacity
+    // bank (SoA)
+    double *x, *y, *z, *u, *v, *w, *E, *wgt, *sigt;
+    uint64_t *rng, *hid;
+    int32_t *pcell, *pprob, *psurf;
+    uint32_t* nflight;
+    uint8_t* status;
+    double* hist;
+    double* S;                    // likelihood-ratio scores [slot][param]
+    double* G;                    // per-history gradient  [slot][param]
+    uint32_t *qA, *qS, *qC;
+    Counters* ctr;
+    // tallies
+    double* sum;
+    double* sq;
+    const double* gout;           // gradient weights per bin
+    double* gsum;
+    double* gsq;
+    mc_trace_record* trace;       // device buffer [prob][trace_count]
+    uint64_t trace_count;
+};
+
+// ------------------------------------------------------------------ XS lookup
+
+// Nuclide::checkE's binary_search (Algorithm.cpp:46-64) narrowed by the
+// grid's bucket table: the result equals the full search exactly.
+__device__ __forceinline__ int grid_index(const KArgs& A, const DGrid& g, double E)
+{
+    const int64_t k = (int64_t)(double_to_bits(E) >> kBucketShift) - g.key_base;
+    if (k < 0) return -1;
+    if (k >= g.nb) return g.n - 1;
+    const int32_t* b = A.bucket + g.b_off + k;
+    int left = b[0] + 1, right = b[1];
+    const double* e = A.grid_E + g.e_off;
+    while (left <= right) {
+        const int mid = (left + right) >> 1;
+        if (e[mid] < E) left = mid + 1;
+        else right = mid - 1;
+    }
+    return right;
+}
+
+// Interpolation point on one grid, reused by every nuclide sharing the grid.
+struct XsPoint {
+    int grid = -1;
+    int idx = 0, n = 0;
+    double a = 0.0, b = 0.0;   // (E-E2)/(E1-E2) and (E-E1)/(E2-E1)
+};
+
+__device__ __forceinline__ void xs_point(const KArgs& A, int gid, double E, XsPoint& P)
+{
+    if (P.grid == gid) return;
+    const DGrid g = A.grid[gid];
+    P.grid = gid;
+    P.n = g.n;
+    P.idx = grid_index(A, g, E);
+    if (P.idx >= 0 && P.idx < g.n - 1) {
+        const double E1 = A.grid_E[g.e_off + P.idx], E2 = A.grid_E[g.e_off + P.idx + 1];
+        P.a = (E - E2) / (E1 - E2);
+        P.b = (E - E1) / (E2 - E1);
+    }
+}
+
+// XSTable::xs (XSec.cpp:9-40) with the interpolation of Algorithm.cpp:103-105
+__device__ __forceinline__ double xs_table(const double* y, const XsPoint& P)
+{
+    if (P.idx == P.n - 1) return y[P.n - 1];
+    if (P.idx == -1) return y[0];
+    return P.a * y[P.idx] + P.b * y[P.idx + 1];
+}
+
+enum { XS_T = 0, XS_S = 1, XS_C = 2 };
+
+__device__ __forceinline__ double nuclide_xs(const KArgs& A, int n, int which, double E, XsPoint& P)
+{
+    const DNuc N = A.nuc[n];
+    if (N.kind == MC_NUC_CONSTANT) return which == XS_S ? N.xs_s : which == XS_C ? N.xs_c : N.xs_t;
+    xs_point(A, N.grid, E, P);
+    const double* t = which == XS_S ? A.tab_s : which == XS_C ? A.tab_c : A.tab_t;
+    return xs_table(t + N.tab, P);
+}
+
+struct Xs3 {
+    double t, s, c;
+};
+
+// Material::SigmaT/S/C (Material.cpp:18-57); parameterised materials use the
+// per-reaction densities and Sigma_t = Sigma_s + Sigma_c.
+__device__ __forceinline__ double material_sigt(const KArgs& A, int m, double E)
+{
+    const mc_material_desc M = A.mat[m];
+    XsPoint P;
+    if (M.param_count == 0) {
+        double t = 0.0;
+        for (int k = 0; k < M.nuc_count; ++k) {
+            const int j = M.nuc_begin + k;
+            t += nuclide_xs(A, A.mat_nuc[j], XS_T, E, P) * A.mat_dens[j];
+        }
+        return t;
+    }
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < M.nuc_count; ++k) {
+        const int j = M.nuc_begin + k;
+        s += nuclide_xs(A, A.mat_nuc[j], XS_S, E, P) * A.mat_ns[j];
+        c += nuclide_xs(A, A.mat_nuc[j], XS_C, E, P) * A.mat_nc[j];
+    }
+    return s + c;
+}
+
+__device__ __forceinline__ Xs3 material_xs3(const KArgs& A, int m, double E)
+{
+    const mc_material_desc M = A.mat[m];
+    XsPoint P;
+    Xs3 r = {0.0, 0.0, 0.0};
+    if (M.param_count == 0) {
+        for (int k = 0; k < M.nuc_count; ++k) {
+            const int j = M.nuc_begin + k, n = A.mat_nuc[j];
+            const double N = A.mat_dens[j];
+            r.t += nuclide_xs(A, n, XS_T, E, P) * N;
+            r.s += nuclide_xs(A, n, XS_S, E, P) * N;
+            r.c += nuclide_xs(A, n, XS_C, E, P) * N;
+        }
+    } else {
+        for (int k = 0; k < M.nuc_count; ++k) {
+            const int j = M.nuc_begin + k, n = A.mat_nuc[j];
+            r.s += nuclide_xs(A, n, XS_S, E, P) * A.mat_ns[j];
+            r.c += nuclide_xs(A, n, XS_C, E, P) * A.mat_nc[j];
+        }
+        r.t = r.s + r.c;
+    }
+    return r;
+}
+
+// d Sigma_s / d theta_k and d Sigma_c / d theta_k for parameter q of material m
+struct DSig {
+    int k;
+    double ds, dc;
+};
+
+__device__ __forceinline__ DSig param_dsig(const KArgs& A, const mc_material_desc& M, int q, double E, XsPoint& P)
+{
+    DSig r;
+    r.k = A.mat_param[M.param_begin + q];
+    const mc_param_desc D = A.param[r.k];
+    const int j = M.nuc_begin + D.nuc_slot, n = A.mat_nuc[j];
+    const double N = A.mat_dens[j];
+    r.ds = (D.reactions & MC_REACTION_SCATTER) ? nuclide_xs(A, n, XS_S, E, P) * N : 0.0;
+    r.dc = (D.reactions & MC_REACTION_CAPTURE) ? nuclide_xs(A, n, XS_C, E, P) * N : 0.0;
+    return r;
+}
+
+// ------------------------------------------------------------------ wave ops
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Ballot + popcount compaction: one atomic per wave, ranks in lane order.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int lane = (int)lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+}  // namespace mc

@@ -1,0 +1,775 @@
+// mc_engine.hip — host side of the GPU engine: device upload of one or many
+// problems, the event loop (hipGraph replay of captured iterations), tallies,
+// gradients, the standalone XS lookup and the launch-shape autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mc_device.hpp"
+#include "mc_engine.hpp"
+#include "mc_problem.hpp"
+
+namespace mc {
+
+// kernels (mc_kernels.hip)
+__global__ void k_reset(Counters* c);
+template <bool kGrad> __global__ void k_refill(KArgs A);
+__global__ void k_xs_lookup(KArgs A);
+template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
+template <bool kGrad> __global__ void k_surface(KArgs A);
+template <bool kGrad> __global__ void k_collision(KArgs A);
+__global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
+__global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
+                            const int32_t* mats, int32_t nmat, uint64_t seed);
+__global__ void k_fill_f64(double* p, uint64_t n, double v);
+__global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v);
+__global__ void k_selftest_math(const double* in, double* out, uint64_t n);
+
+#define HIP_CHECK(x)                                                                          \
+    do {                                                                                      \
+        hipError_t _e = (x);                                                                  \
+        if (_e != hipSuccess)                                                                 \
+            throw Error(MC_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(_e));      \
+    } while (0)
+
+// ----------------------------------------------------------- device buffers
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    void alloc(size_t n)
+    {
+        if (n <= bytes && p) return;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (n == 0) n = 8;
+        HIP_CHECK(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+template <class T>
+static void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st)
+{
+    b.alloc(sizeof(T) * v.size());
+    if (!v.empty()) HIP_CHECK(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+}
+
+static uint64_t lcg_pow(uint64_t g, uint64_t e)
+{
+    uint64_t r = 1;
+    for (; e; e >>= 1) {
+        if (e & 1) r = (r * g) & kRnMask;
+        g = (g * g) & kRnMask;
+    }
+    return r;
+}
+
+struct Engine::Impl {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    mc_engine_cfg cfg{};
+    std::vector<const Problem*> probs;
+
+    // concatenated host data
+    std::vector<mc_surface_desc> surf;
+    std::vector<mc_cell_desc> cell;
+    std::vector<int32_t> cell_surf, cell_sense, nbr_begin, nbr;
+    std::vector<mc_material_desc> mat;
+    std::vector<int32_t> mat_nuc, mat_param;
+    std::vector<double> mat_dens;
+    std::vector<mc_param_desc> param;
+    std::vector<DNuc> nuc;
+    std::vector<DGrid> grid;
+    std::vector<double> grid_E, tab_s, tab_c, tab_t;
+    std::vector<int32_t> bucket;
+    std::vector<mc_estimator_desc> est;
+    std::vector<int32_t> att_est, att_idx;
+    std::vector<double> egrid;
+    std::vector<DProb> prob;
+    std::vector<mc_source_desc> src;
+    std::vector<double> src_cdf;
+    std::vector<int32_t> cdf_off;
+    std::vector<int32_t> mat_off;      // per problem: first material index
+    std::vector<uint64_t> jump;
+    int n_bins = 0, n_params = 0, bin_stride = 1, par_stride = 1;
+    bool has_tables = false;
+    double emin = 0, emax = 0;
+
+    // device data
+    DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
+        d_mat_ns, d_mat_nc, d_mat_param, d_param, d_theta, d_nuc, d_grid, d_grid_E, d_bucket, d_tab_s,
+        d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
+        d_run_hb;
+    // bank
+    DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
+        b_status, b_hist, b_S, b_G, q_A, q_S, q_C, d_ctr;
+    uint64_t bank_cap = 0;
+    bool bank_grad = false;
+    // tallies
+    DevBuf t_sum, t_sq, t_gout, t_gsum, t_gsq, t_trace;
+    Counters* h_ctr = nullptr;   // pinned
+    // graphs
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_iters[2] = {0, 0};
+    uint64_t generation = 0, graph_gen[2] = {~0ull, ~0ull};
+    const void* graph_trace[2] = {nullptr, nullptr};   // trace pointer captured in each graph
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~Impl()
+    {
+        for (auto& g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        if (h_ctr) (void)hipHostFree(h_ctr);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void build(const std::vector<const Problem*>& ps);
+    void upload_static();
+    void ensure_bank(bool grad);
+    KArgs args(bool grad, mc_trace_record* dtrace, uint64_t trace_count) const;
+    void launch_iteration(const KArgs& A, bool grad, hipStream_t st);
+    void run_loop(bool grad, const KArgs& A);
+    void run(const mc_run_cfg* run, bool grad, const double* gout, mc_tally_out* out, double* grad_mean,
+             double* grad_sigma);
+    dim3 grid_dim() const { return dim3((unsigned)std::max(1, n_cu * cfg.grid_per_cu)); }
+};
+
+// ---------------------------------------------------------------- building
+void Engine::Impl::build(const std::vector<const Problem*>& ps)
+{
+    probs = ps;
+    std::map<std::vector<double>, int> grid_ids;
+    std::map<std::string, int> nuc_ids;
+    for (size_t p = 0; p < ps.size(); ++p) {
+        const Problem& P = *ps[p];
+        const mc_problem_desc& d = P.desc;
+        const int surf_off = (int)surf.size(), cell_off = (int)cell.size(), cs_off = (int)cell_surf.size();
+        const int mat_o = (int)mat.size(), mn_off = (int)mat_nuc.size(), est_off = (int)est.size();
+        const int att_off = (int)att_est.size(), eg_off = (int)egrid.size(), src_off = (int)src.size();
+        const int par_off = (int)param.size(), mp_off = (int)mat_param.size();
+        // nuclides: deduplicated by content
+        std::vector<int> nmap(d.n_nuclides);
+        for (int n = 0; n < d.n_nuclides; ++n) {
+            const mc_nuclide_desc& N = d.nuclides[n];
+            std::string key((const char*)&N.A, sizeof(double));
+            key.append((const char*)&N.kind, sizeof(int32_t));
+            key.append((const char*)&N.xs_scatter, 3 * sizeof(double));
+            if (N.kind == MC_NUC_TABLE) {
+                key.append((const char*)(d.xs_energy + N.grid_begin), sizeof(double) * N.grid_count);
+                key.append((const char*)(d.xs_scatter + N.grid_begin), sizeof(double) * N.grid_count);
+                key.append((const char*)(d.xs_capture + N.grid_begin), sizeof(double) * N.grid_count);
+                key.append((const char*)(d.xs_total + N.grid_begin), sizeof(double) * N.grid_count);
+            }
+            auto it = nuc_ids.find(key);
+            if (it != nuc_ids.end()) { nmap[n] = it->second; continue; }
+            DNuc D{};
+            D.A = N.A;
+            D.kind = N.kind;
+            D.xs_s = N.xs_scatter;
+            D.xs_c = N.xs_capture;
+            D.xs_t = N.xs_total;
+            D.grid = -1;
+            if (N.kind == MC_NUC_TABLE) {
+                has_tables = true;
+                std::vector<double> E(d.xs_energy + N.grid_begin, d.xs_energy + N.grid_begin + N.grid_count);
+                auto g = grid_ids.find(E);
+                if (g == grid_ids.end()) {
+                    DGrid G{};
+                    G.e_off = (int)grid_E.size();
+                    G.n = (int)E.size();
+                    grid_E.insert(grid_E.end(), E.begin(), E.end());
+                    // bucket table over the bit pattern: lo[k] = binary_search(low(k))
+                    const int64_t k0 = (int64_t)(double_to_bits(E.front()) >> kBucketShift);
+                    const int64_t k1 = (int64_t)(double_to_bits(E.back()) >> kBucketShift);
+                    G.key_base = k0;
+                    G.nb = (int)(k1 - k0 + 1);
+                    G.b_off = (int)bucket.size();
+                    for (int64_t k = 0; k <= G.nb; ++k) {
+                        const double low = bits_to_double((uint64_t)(k0 + k) << kBucketShift);
+                        bucket.push_back(binary_search(low, E.data(), (int)E.size()));
+                    }
+                    grid.push_back(G);
+                    g = grid_ids.emplace(E, (int)grid.size() - 1).first;
+                    if (emin == 0 || E.front() < emin) emin = E.front();
+                    emax = std::max(emax, E.back());
+                }
+                D.grid = g->second;
+                D.tab = (int)tab_s.size();
+                D.n = N.grid_count;
+                tab_s.insert(tab_s.end(), d.xs_scatter + N.grid_begin, d.xs_scatter + N.grid_begin + N.grid_count);
+                tab_c.insert(tab_c.end(), d.xs_capture + N.grid_begin, d.xs_capture + N.grid_begin + N.grid_count);
+                tab_t.insert(tab_t.end(), d.xs_total + N.grid_begin, d.xs_total + N.grid_begin + N.grid_count);
+            }
+            nuc.push_back(D);
+            nmap[n] = (int)nuc.size() - 1;
+            nuc_ids.emplace(key, nmap[n]);
+        }
+        for (int s = 0; s < d.n_surfaces; ++s) {
+            mc_surface_desc S = d.surfaces[s];
+            S.est_begin += att_off;
+            surf.push_back(S);
+        }
+        for (int c = 0; c < d.n_cells; ++c) {
+            mc_cell_desc C = d.cells[c];
+            if (C.material >= 0) C.material += mat_o;
+            C.surf_begin += cs_off;
+            C.tl_begin += att_off;
+            C.col_begin += att_off;
+            cell.push_back(C);
+        }
+        int ncs = 0;
+        for (int c = 0; c < d.n_cells; ++c) ncs = std::max(ncs, d.cells[c].surf_begin + d.cells[c].surf_count);
+        for (int k = 0; k < ncs; ++k) {
+            cell_surf.push_back(d.cell_surf[k] + surf_off);
+            cell_sense.push_back(d.cell_sense[k]);
+        }
+        int nmn = 0;
+        for (int m = 0; m < d.n_materials; ++m) {
+            mc_material_desc M = d.materials[m];
+            nmn = std::max(nmn, M.nuc_begin + M.nuc_count);
+            M.nuc_begin += mn_off;
+            M.param_begin += mp_off;
+            mat.push_back(M);
+        }
+        for (int k = 0; k < nmn; ++k) {
+            mat_nuc.push_back(nmap[d.mat_nuclide[k]]);
+            mat_dens.push_back(d.mat_density[k]);
+        }
+        for (int k = 0; k < d.n_params; ++k) {
+            mc_param_desc D = d.params[k];
+            D.material += mat_o;
+            D.cell += cell_off;
+            param.push_back(D);
+        }
+        int nmp = 0;
+        for (int m = 0; m < d.n_materials; ++m)
+            nmp = std::max(nmp, d.materials[m].param_begin + d.materials[m].param_count);
+        for (int k = 0; k < nmp; ++k) mat_param.push_back(d.mat_param[k] + par_off);
+        for (int e = 0; e < d.n_estimators; ++e) {
+            mc_estimator_desc E = d.estimators[e];
+            E.egrid_begin += eg_off;
+            E.bin_offset += n_bins;
+            est.push_back(E);
+        }
+        int natt = 0;
+        for (int s = 0; s < d.n_surfaces; ++s) natt = std::max(natt, d.surfaces[s].est_begin + d.surfaces[s].est_count);
+        for (int c = 0; c < d.n_cells; ++c) {
+            natt = std::max(natt, d.cells[c].tl_begin + d.cells[c].tl_count);
+            natt = std::max(natt, d.cells[c].col_begin + d.cells[c].col_count);
+        }
+        for (int k = 0; k < natt; ++k) {
+            att_est.push_back(d.att_estimator[k] + est_off);
+            att_idx.push_back(d.att_index[k]);
+        }
+        int neg = 0;
+        for (int e = 0; e < d.n_estimators; ++e)
+            neg = std::max(neg, d.estimators[e].egrid_begin + d.estimators[e].egrid_count);
+        egrid.insert(egrid.end(), d.egrid, d.egrid + neg);
+        for (int s = 0; s < d.n_sources; ++s) {
+            mc_source_desc S = d.sources[s];
+            S.cell += cell_off;
+            src.push_back(S);
+        }
+        cdf_off.push_back((int)src_cdf.size());
+        src_cdf.insert(src_cdf.end(), d.source_cdf, d.source_cdf + d.n_sources + 1);
+        DProb R{};
+        R.cell_begin = cell_off;
+        R.cell_count = d.n_cells;
+        R.src_begin = src_off;
+        R.src_count = d.n_sources;
+        R.bin_begin = n_bins;
+        R.bin_count = d.n_bins;
+        R.param_begin = par_off;
+        R.param_count = d.n_params;
+        R.seed = d.seed;
+        R.wr = d.wr;
+        R.ws = d.ws;
+        prob.push_back(R);
+        mat_off.push_back(mat_o);
+        n_bins += d.n_bins;
+        n_params += d.n_params;
+        bin_stride = std::max(bin_stride, d.n_bins);
+        par_stride = std::max(par_stride, d.n_params);
+    }
+    // per surface: cells that list it, in index order
+    std::vector<std::vector<int>> lists(surf.size());
+    for (size_t c = 0; c < cell.size(); ++c)
+        for (int k = 0; k < cell[c].surf_count; ++k) {
+            auto& L = lists[cell_surf[cell[c].surf_begin + k]];
+            if (L.empty() || L.back() != (int)c) L.push_back((int)c);
+        }
+    nbr_begin.push_back(0);
+    for (auto& L : lists) {
+        nbr.insert(nbr.end(), L.begin(), L.end());
+        nbr_begin.push_back((int)nbr.size());
+    }
+    // LCG jump multipliers g^(stride * 2^k)
+    jump.resize(64);
+    jump[0] = lcg_pow(kRnMult, kRnStride);
+    for (int k = 1; k < 64; ++k) jump[k] = (jump[k - 1] * jump[k - 1]) & kRnMask;
+}
+
+void Engine::Impl::upload_static()
+{
+    upload(d_surf, surf, stream);
+    upload(d_cell, cell, stream);
+    upload(d_cell_surf, cell_surf, stream);
+    upload(d_cell_sense, cell_sense, stream);
+    upload(d_nbr_begin, nbr_begin, stream);
+    upload(d_nbr, nbr, stream);
+    upload(d_mat, mat, stream);
+    upload(d_mat_nuc, mat_nuc, stream);
+    upload(d_mat_dens, mat_dens, stream);
+    upload(d_mat_ns, mat_dens, stream);
+    upload(d_mat_nc, mat_dens, stream);
+    upload(d_mat_param, mat_param, stream);
+    upload(d_param, param, stream);
+    upload(d_theta, std::vector<double>(std::max(1, n_params), 1.0), stream);
+    upload(d_nuc, nuc, stream);
+    upload(d_grid, grid, stream);
+    upload(d_grid_E, grid_E, stream);
+    upload(d_bucket, bucket, stream);
+    upload(d_tab_s, tab_s, stream);
+    upload(d_tab_c, tab_c, stream);
+    upload(d_tab_t, tab_t, stream);
+    upload(d_est, est, stream);
+    upload(d_att_est, att_est, stream);
+    upload(d_att_idx, att_idx, stream);
+    upload(d_egrid, egrid, stream);
+    upload(d_prob, prob, stream);
+    upload(d_src, src, stream);
+    upload(d_src_cdf, src_cdf, stream);
+    upload(d_cdf_off, cdf_off, stream);
+    upload(d_jump, jump, stream);
+    d_run_hb.alloc(sizeof(uint64_t) * (prob.size() + 1));
+    t_sum.alloc(sizeof(double) * std::max(1, n_bins));
+    t_sq.alloc(sizeof(double) * std::max(1, n_bins));
+    t_gout.alloc(sizeof(double) * std::max(1, n_bins));
+    t_gsum.alloc(sizeof(double) * std::max(1, n_params));
+    t_gsq.alloc(sizeof(double) * std::max(1, n_params));
+    d_ctr.alloc(sizeof(Counters));
+    HIP_CHECK(hipStreamSynchronize(stream));
+}
+
+void Engine::Impl::ensure_bank(bool grad)
+{
+    const uint64_t B = cfg.bank_capacity;
+    if (bank_cap == B && (!grad || bank_grad)) return;
+    const bool realloc_all = bank_cap != B;
+    if (realloc_all) {
+        for (DevBuf* b : {&b_x, &b_y, &b_z, &b_u, &b_v, &b_w, &b_E, &b_wgt, &b_sigt}) b->alloc(8 * B);
+        b_rng.alloc(8 * B);
+        b_hid.alloc(8 * B);
+        b_cell.alloc(4 * B);
+        b_prob.alloc(4 * B);
+        b_surf.alloc(4 * B);
+        b_nfl.alloc(4 * B);
+        b_status.alloc(B);
+        b_hist.alloc(8 * B * (uint64_t)bin_stride);
+        q_A.alloc(4 * B);
+        q_S.alloc(4 * B);
+        q_C.alloc(4 * B);
+        bank_cap = B;
+        bank_grad = false;
+    }
+    if (grad && (!bank_grad || realloc_all)) {
+        b_S.alloc(8 * B * (uint64_t)par_stride);
+        b_G.alloc(8 * B * (uint64_t)par_stride);
+        bank_grad = true;
+    }
+    ++generation;
+}
+
+KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_count) const
+{
+    KArgs A{};
+    A.surf = d_surf.as<mc_surface_desc>();
+    A.cell = d_cell.as<mc_cell_desc>();
+    A.cell_surf = d_cell_surf.as<int32_t>();
+    A.cell_sense = d_cell_sense.as<int32_t>();
+    A.nbr_begin = d_nbr_begin.as<int32_t>();
+    A.nbr = d_nbr.as<int32_t>();
+    A.mat = d_mat.as<mc_material_desc>();
+    A.mat_nuc = d_mat_nuc.as<int32_t>();
+    A.mat_dens = d_mat_dens.as<double>();
+    A.mat_ns = d_mat_ns.as<double>();
+    A.mat_nc = d_mat_nc.as<double>();
+    A.mat_param = d_mat_param.as<int32_t>();
+    A.param = d_param.as<mc_param_desc>();
+    A.theta = d_theta.as<double>();
+    A.nuc = d_nuc.as<DNuc>();
+    A.grid = d_grid.as<DGrid>();
+    A.grid_E = d_grid_E.as<double>();
+    A.bucket = d_bucket.as<int32_t>();
+    A.tab_s = d_tab_s.as<double>();
+    A.tab_c = d_tab_c.as<double>();
+    A.tab_t = d_tab_t.as<double>();
+    A.est = d_est.as<mc_estimator_desc>();
+    A.att_est = d_att_est.as<int32_t>();
+    A.att_idx = d_att_idx.as<int32_t>();
+    A.egrid = d_egrid.as<double>();
+    A.prob = d_prob.as<DProb>();
+    A.src = d_src.as<mc_source_desc>();
+    A.src_cdf = d_src_cdf.as<double>();
+    A.cdf_off = d_cdf_off.as<int32_t>();
+    A.jump = d_jump.as<uint64_t>();
+    A.run_hb = d_run_hb.as<uint64_t>();
+    A.n_prob = (int32_t)prob.size();
+    A.bin_stride = bin_stride;
+    A.par_stride = par_stride;
+    A.cap = bank_cap;
+    A.x = b_x.as<double>(); A.y = b_y.as<double>(); A.z = b_z.as<double>();
+    A.u = b_u.as<double>(); A.v = b_v.as<double>(); A.w = b_w.as<double>();
+    A.E = b_E.as<double>();
+    A.wgt = b_wgt.as<double>();
+    A.sigt = b_sigt.as<double>();
+    A.rng = b_rng.as<uint64_t>();
+    A.hid = b_hid.as<uint64_t>();
+    A.pcell = b_cell.as<int32_t>();
+    A.pprob = b_prob.as<int32_t>();
+    A.psurf = b_surf.as<int32_t>();
+    A.nflight = b_nfl.as<uint32_t>();
+    A.status = b_status.as<uint8_t>();
+    A.hist = b_hist.as<double>();
+    A.S = grad ? b_S.as<double>() : nullptr;
+    A.G = grad ? b_G.as<double>() : nullptr;
+    A.qA = q_A.as<uint32_t>();
+    A.qS = q_S.as<uint32_t>();
+    A.qC = q_C.as<uint32_t>();
+    A.ctr = d_ctr.as<Counters>();
+    A.sum = t_sum.as<double>();
+    A.sq = t_sq.as<double>();
+    A.gout = t_gout.as<double>();
+    A.gsum = t_gsum.as<double>();
+    A.gsq = t_gsq.as<double>();
+    A.trace = dtrace;
+    A.trace_count = trace_count;
+    return A;
+}
+
+void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st)
+{
+    const dim3 g = grid_dim(), b((unsigned)cfg.block);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, A.ctr);
+    if (grad) {
+        hipLaunchKernelGGL(k_refill<true>, g, b, 0, st, A);
+        if (has_tables) {
+            hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
+            hipLaunchKernelGGL((k_flight<true, true>), g, b, 0, st, A);
+        } else {
+            hipLaunchKernelGGL((k_flight<false, true>), g, b, 0, st, A);
+        }
+        hipLaunchKernelGGL(k_surface<true>, g, b, 0, st, A);
+        hipLaunchKernelGGL(k_collision<true>, g, b, 0, st, A);
+    } else {
+        hipLaunchKernelGGL(k_refill<false>, g, b, 0, st, A);
+        if (has_tables) {
+            hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
+            hipLaunchKernelGGL((k_flight<true, false>), g, b, 0, st, A);
+        } else {
+            hipLaunchKernelGGL((k_flight<false, false>), g, b, 0, st, A);
+        }
+        hipLaunchKernelGGL(k_surface<false>, g, b, 0, st, A);
+        hipLaunchKernelGGL(k_collision<false>, g, b, 0, st, A);
+    }
+}
+
+void Engine::Impl::run_loop(bool grad, const KArgs& A)
+{
+    const int gi = grad ? 1 : 0;
+    const int K = std::max(1, cfg.iters_per_graph);
+    if (cfg.use_graph) {
+        if (!graph[gi] || graph_gen[gi] != generation || graph_iters[gi] != K ||
+            graph_trace[gi] != (const void*)A.trace) {
+            if (graph[gi]) { HIP_CHECK(hipGraphExecDestroy(graph[gi])); graph[gi] = nullptr; }
+            hipGraph_t gr;
+            HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream);
+            HIP_CHECK(hipStreamEndCapture(stream, &gr));
+            HIP_CHECK(hipGraphInstantiate(&graph[gi], gr, nullptr, nullptr, 0));
+            HIP_CHECK(hipGraphDestroy(gr));
+            graph_gen[gi] = generation;
+            graph_iters[gi] = K;
+            graph_trace[gi] = (const void*)A.trace;
+        }
+    }
+    for (uint64_t chunk = 0;; ++chunk) {
+        if (cfg.use_graph) HIP_CHECK(hipGraphLaunch(graph[gi], stream));
+        else
+            for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        if (h_ctr->nA == 0) break;
+        if (chunk > (1ull << 40)) throw Error(MC_ERR_DEVICE, "event loop did not terminate");
+    }
+}
+
+void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_tally_out* out,
+                       double* grad_mean, double* grad_sigma)
+{
+    HIP_CHECK(hipSetDevice(device));
+    const int np = (int)prob.size();
+    std::vector<uint64_t> nh(np), hb(np + 1, 0);
+    for (int p = 0; p < np; ++p) {
+        nh[p] = (run && run->n_histories) ? run->n_histories[p] : probs[p]->n_histories;
+        if (nh[p] < 2) throw Error(MC_ERR_ARGUMENT, "each problem needs at least 2 histories");
+        hb[p + 1] = hb[p] + nh[p];
+    }
+    ensure_bank(grad);
+    // theta -> per-reaction densities (as the oracle: N_s = theta * N)
+    std::vector<double> th(std::max(1, n_params), 1.0);
+    if (run && run->theta)
+        for (int k = 0; k < n_params; ++k) th[k] = run->theta[k];
+    std::vector<double> ns(mat_dens), nc(mat_dens);
+    for (int k = 0; k < n_params; ++k) {
+        const mc_param_desc& D = param[k];
+        const int j = mat[D.material].nuc_begin + D.nuc_slot;
+        if (D.reactions & MC_REACTION_SCATTER) ns[j] = th[k] * mat_dens[j];
+        if (D.reactions & MC_REACTION_CAPTURE) nc[j] = th[k] * mat_dens[j];
+    }
+    upload(d_theta, th, stream);
+    upload(d_mat_ns, ns, stream);
+    upload(d_mat_nc, nc, stream);
+    HIP_CHECK(hipMemcpyAsync(d_run_hb.p, hb.data(), sizeof(uint64_t) * (np + 1), hipMemcpyHostToDevice, stream));
+    if (grad) {
+        std::vector<double> g(gout, gout + n_bins);
+        HIP_CHECK(hipMemcpyAsync(t_gout.p, g.data(), sizeof(double) * n_bins, hipMemcpyHostToDevice, stream));
+    }
+    // trace buffer
+    mc_trace_record* dtrace = nullptr;
+    const uint64_t tcount = (run && run->trace) ? run->trace_count : 0;
+    if (tcount) {
+        const size_t bytes = sizeof(mc_trace_record) * tcount * (size_t)np;
+        if (t_trace.bytes < bytes) { t_trace.alloc(bytes); ++generation; }
+        HIP_CHECK(hipMemsetAsync(t_trace.p, 0, bytes, stream));
+        dtrace = t_trace.as<mc_trace_record>();
+    }
+    const KArgs A = args(grad, dtrace, tcount);
+    const dim3 fg(1024), fb(256);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.sum, (uint64_t)std::max(1, n_bins), 0.0);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.sq, (uint64_t)std::max(1, n_bins), 0.0);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.hist, bank_cap * (uint64_t)bin_stride, 0.0);
+    if (grad) {
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.gsum, (uint64_t)std::max(1, n_params), 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.gsq, (uint64_t)std::max(1, n_params), 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.G, bank_cap * (uint64_t)par_stride, 0.0);
+    }
+    HIP_CHECK(hipEventRecord(ev0, stream));
+    const bool plain = run && run->mode == MC_RUN_PLAIN;
+    uint64_t flights = 0;
+    uint32_t err_lost = 0, err_split = 0;
+    for (int p = 0; p < (plain ? np : 1); ++p) {
+        Counters c{};
+        c.next_hist = plain ? hb[p] : 0;
+        c.end_hist = plain ? hb[p + 1] : hb[np];
+        HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.status, bank_cap, (uint8_t)ST_EMPTY);
+        run_loop(grad, A);
+        flights += h_ctr->flights;
+        err_lost += h_ctr->err_lost;
+        err_split += h_ctr->err_split;
+    }
+    HIP_CHECK(hipEventRecord(ev1, stream));
+    HIP_CHECK(hipEventSynchronize(ev1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    if (err_lost) throw Error(MC_ERR_LOST_PARTICLE, std::to_string(err_lost) + " particle(s) lost");
+    if (err_split)
+        throw Error(MC_ERR_UNSUPPORTED, "importance splitting (ratio > 1) is not supported by the GPU engine");
+    std::vector<double> sum(n_bins), sq(n_bins);
+    HIP_CHECK(hipMemcpy(sum.data(), A.sum, sizeof(double) * n_bins, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(sq.data(), A.sq, sizeof(double) * n_bins, hipMemcpyDeviceToHost));
+    if (out) {
+        out->flights = flights;
+        out->seconds = ms * 1e-3;
+        for (int p = 0; p < np; ++p) {
+            const double N = (double)nh[p];
+            for (int b = prob[p].bin_begin; b < prob[p].bin_begin + prob[p].bin_count; ++b) {
+                if (out->sum) out->sum[b] = sum[b];
+                if (out->sq) out->sq[b] = sq[b];
+                // Estimator::end_cycle / end_simulation, Estimator.cpp:347-367
+                const double mean = sum[b] / N;
+                const double u2 = (sq[b] / N - mean * mean) / (N - 1.0);
+                if (out->mean) out->mean[b] = mean;
+                if (out->uncer) out->uncer[b] = std::sqrt(u2);
+            }
+        }
+    }
+    if (grad) {
+        std::vector<double> gs(n_params), gq(n_params);
+        HIP_CHECK(hipMemcpy(gs.data(), A.gsum, sizeof(double) * n_params, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(gq.data(), A.gsq, sizeof(double) * n_params, hipMemcpyDeviceToHost));
+        for (int p = 0; p < np; ++p) {
+            const double N = (double)nh[p];
+            for (int k = prob[p].param_begin; k < prob[p].param_begin + prob[p].param_count; ++k) {
+                const double m = gs[k] / N;
+                if (grad_mean) grad_mean[k] = m;
+                if (grad_sigma) grad_sigma[k] = std::sqrt((gq[k] / N - m * m) / (N - 1.0));
+            }
+        }
+    }
+    if (tcount) {
+        HIP_CHECK(hipMemcpy(run->trace, dtrace, sizeof(mc_trace_record) * tcount * (size_t)np,
+                            hipMemcpyDeviceToHost));
+    }
+}
+
+// ---------------------------------------------------------------- Engine API
+Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : d(new Impl)
+{
+    if (ps.empty()) throw Error(MC_ERR_ARGUMENT, "no problems");
+    mc_engine_cfg cfg{};
+    if (c) cfg = *c;
+    if (cfg.block == 0) cfg.block = 256;
+    if (cfg.grid_per_cu == 0) cfg.grid_per_cu = 4;
+    if (cfg.iters_per_graph == 0) cfg.iters_per_graph = 16;
+    if (cfg.bank_capacity == 0) cfg.bank_capacity = 1ull << 20;
+    if (cfg.block % 64 || cfg.block > 1024) throw Error(MC_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
+    if (cfg.bank_capacity > (1ull << 31)) throw Error(MC_ERR_CAPACITY, "bank capacity above 2^31");
+    d->cfg = cfg;
+    d->device = cfg.device;
+    int ndev = 0;
+    HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (ndev == 0) throw Error(MC_ERR_DEVICE, "no HIP device");
+    HIP_CHECK(hipSetDevice(d->device));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, d->device));
+    d->n_cu = prop.multiProcessorCount;
+    HIP_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipHostMalloc((void**)&d->h_ctr, sizeof(Counters), hipHostMallocDefault));
+    HIP_CHECK(hipEventCreate(&d->ev0));
+    HIP_CHECK(hipEventCreate(&d->ev1));
+    d->build(ps);
+    d->upload_static();
+}
+
+Engine::~Engine() = default;
+
+void Engine::info(mc_engine_info* o) const
+{
+    o->n_problems = (int32_t)d->prob.size();
+    o->n_bins = d->n_bins;
+    o->n_params = d->n_params;
+    o->bank_capacity = d->cfg.bank_capacity;
+    o->block = d->cfg.block;
+    o->grid_per_cu = d->cfg.grid_per_cu;
+    o->iters_per_graph = d->cfg.iters_per_graph;
+    o->n_cu = d->n_cu;
+}
+
+void Engine::offsets(int p, int32_t* bin_off, int32_t* nb, int32_t* par_off, int32_t* npar) const
+{
+    if (p < 0 || p >= (int)d->prob.size()) throw Error(MC_ERR_ARGUMENT, "problem index out of range");
+    if (bin_off) *bin_off = d->prob[p].bin_begin;
+    if (nb) *nb = d->prob[p].bin_count;
+    if (par_off) *par_off = d->prob[p].param_begin;
+    if (npar) *npar = d->prob[p].param_count;
+}
+
+void Engine::forward(const mc_run_cfg* run, mc_tally_out* out) { d->run(run, false, nullptr, out, nullptr, nullptr); }
+
+void Engine::backward(const mc_run_cfg* run, const double* gout, double* grad, double* sig, mc_tally_out* out)
+{
+    if (!gout) throw Error(MC_ERR_ARGUMENT, "grad_out is required");
+    if (d->n_params == 0) throw Error(MC_ERR_ARGUMENT, "the problems declare no <parameters>");
+    d->run(run, true, gout, out, grad, sig);
+}
+
+void Engine::xs_lookup(const double* E, const int32_t* mat, double* out, uint64_t n)
+{
+    HIP_CHECK(hipSetDevice(d->device));
+    for (uint64_t i = 0; i < n; ++i)
+        if (mat[i] < -1 || mat[i] >= (int32_t)d->probs[0]->materials.size())
+            throw Error(MC_ERR_ARGUMENT, "material index out of range");
+    DevBuf bE, bM, bO;
+    bE.alloc(8 * n);
+    bM.alloc(4 * n);
+    bO.alloc(8 * n);
+    HIP_CHECK(hipMemcpyAsync(bE.p, E, 8 * n, hipMemcpyHostToDevice, d->stream));
+    HIP_CHECK(hipMemcpyAsync(bM.p, mat, 4 * n, hipMemcpyHostToDevice, d->stream));
+    // problem 0's materials start at index 0
+    const KArgs A = d->args(false, nullptr, 0);
+    hipLaunchKernelGGL(k_xs_lookup_array, d->grid_dim(), dim3(d->cfg.block), 0, d->stream, A, bE.as<double>(),
+                       bM.as<int32_t>(), bO.as<double>(), n);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(out, bO.p, 8 * n, hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+
+void Engine::xs_bench(uint64_t n, int iters, uint64_t seed, double* sec, double* checksum)
+{
+    HIP_CHECK(hipSetDevice(d->device));
+    if (!d->has_tables) throw Error(MC_ERR_ARGUMENT, "no tabulated nuclides to look up");
+    std::vector<int32_t> mats;
+    for (int m = 0; m < (int)d->probs[0]->materials.size(); ++m) mats.push_back(m);
+    DevBuf bE, bM, bO, bmats;
+    bE.alloc(8 * n);
+    bM.alloc(4 * n);
+    bO.alloc(8 * n);
+    upload(bmats, mats, d->stream);
+    const double lnmin = std::log(d->emin), lnspan = std::log(d->emax) - std::log(d->emin);
+    hipLaunchKernelGGL(k_xs_inputs, dim3(2048), dim3(256), 0, d->stream, bE.as<double>(), bM.as<int32_t>(), n,
+                       lnmin, lnspan, bmats.as<int32_t>(), (int32_t)mats.size(), seed);
+    const KArgs A = d->args(false, nullptr, 0);
+    const dim3 g = d->grid_dim(), b(d->cfg.block);
+    hipLaunchKernelGGL(k_xs_lookup_array, g, b, 0, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
+                       bO.as<double>(), n);   // warm-up
+    HIP_CHECK(hipEventRecord(d->ev0, d->stream));
+    for (int k = 0; k < iters; ++k)
+        hipLaunchKernelGGL(k_xs_lookup_array, g, b, 0, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
+                           bO.as<double>(), n);
+    HIP_CHECK(hipEventRecord(d->ev1, d->stream));
+    HIP_CHECK(hipEventSynchronize(d->ev1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    *sec = ms * 1e-3 / std::max(1, iters);
+    if (checksum) {
+        std::vector<double> o(std::min<uint64_t>(n, 1 << 16));
+        HIP_CHECK(hipMemcpy(o.data(), bO.p, 8 * o.size(), hipMemcpyDeviceToHost));
+        double s = 0;
+        for (double v : o) s += v;
+        *checksum = s;
+    }
+}
+
+void Engine::set_cfg(const mc_engine_cfg& c)
+{
+    d->cfg.block = c.block;
+    d->cfg.grid_per_cu = c.grid_per_cu;
+    d->cfg.iters_per_graph = c.iters_per_graph;
+    d->cfg.use_graph = c.use_graph;
+    if (c.bank_capacity != d->cfg.bank_capacity) d->cfg.bank_capacity = c.bank_capacity;
+    ++d->generation;
+}
+
+mc_engine_cfg Engine::cfg() const { return d->cfg; }
+
+int Engine::n_problems() const { return (int)d->prob.size(); }
+uint64_t Engine::deck_histories(int p) const { return d->probs[p]->n_histories; }
+
+void selftest_math(const double* in, double* out, uint64_t n)
+{
+    DevBuf bi, bo;
+    bi.alloc(8 * n);
+    bo.alloc(32 * n);
+    HIP_CHECK(hipMemcpy(bi.p, in, 8 * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_selftest_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, bi.as<double>(),
+                       bo.as<double>(), n);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpy(out, bo.p, 32 * n, hipMemcpyDeviceToHost));
+}
+
+}  // namespace mc

@@ -1,0 +1,528 @@
+// mc_problem.cpp — deck loader in the reference's XML schema.
+//
+// Follows Simulator::Simulator (src/simulator/setup.cpp:30-1069) element by
+// element.  Where the reference exits or crashes, this loader raises an
+// mc::Error with a status code instead.  Extensions (all optional):
+//   <nuclide><scatter xs=".."/></nuclide>      constant scattering XS (one-speed)
+//   <description ... seed=".."/>               LCG seed (default 1, Random.cpp:105)
+//   <parameters><parameter cell=".." nuclide=".." reaction="scatter|capture|total"/>
+//                                              gradient parameters (theta scales)
+#include "mc_problem.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
+
+#include "mc_physics.hpp"
+#include "mc_xml.hpp"
+
+namespace mc {
+
+int ref_binary_search(double x, const double* v, int n) { return binary_search(x, v, n); }
+
+int Problem::search_cell(const double p[3]) const
+{
+    const Vec3 q = {p[0], p[1], p[2]};
+    for (size_t c = 0; c < cells.size(); ++c) {
+        bool inside = true;
+        for (int k = 0; k < cells[c].surf_count; ++k) {
+            const int j = cells[c].surf_begin + k;
+            const mc_surface_desc& S = surfaces[cell_surf[j]];
+            if (surface_eval(S.type, S.c, q) * (double)cell_sense[j] < 0) { inside = false; break; }
+        }
+        if (inside) return (int)c;
+    }
+    return -1;
+}
+
+void Problem::finalize()
+{
+    desc = mc_problem_desc{};
+    desc.n_surfaces = (int32_t)surfaces.size();
+    desc.n_cells = (int32_t)cells.size();
+    desc.n_materials = (int32_t)materials.size();
+    desc.n_nuclides = (int32_t)nuclides.size();
+    desc.n_sources = (int32_t)sources.size();
+    desc.n_estimators = (int32_t)estimators.size();
+    desc.n_params = (int32_t)params.size();
+    desc.n_bins = n_bins;
+    desc.surfaces = surfaces.data();
+    desc.cells = cells.data();
+    desc.cell_surf = cell_surf.data();
+    desc.cell_sense = cell_sense.data();
+    desc.materials = materials.data();
+    desc.mat_nuclide = mat_nuclide.data();
+    desc.mat_density = mat_density.data();
+    desc.nuclides = nuclides.data();
+    desc.xs_energy = xs_energy.data();
+    desc.xs_scatter = xs_scatter.data();
+    desc.xs_capture = xs_capture.data();
+    desc.xs_total = xs_total.data();
+    desc.sources = sources.data();
+    desc.source_cdf = source_cdf.data();
+    desc.estimators = estimators.data();
+    desc.att_estimator = att_estimator.data();
+    desc.att_index = att_index.data();
+    desc.egrid = egrid.data();
+    desc.params = params.data();
+    desc.mat_param = mat_param.data();
+    desc.wr = wr;
+    desc.ws = ws;
+    desc.n_histories = n_histories;
+    desc.seed = seed;
+}
+
+void load_xs_file(const std::string& path, double& A, std::vector<double>& E,
+                  std::vector<double>& s, std::vector<double>& c, std::vector<double>& f)
+{
+    std::ifstream in(path);
+    if (!in) throw Error(MC_ERR_IO, "cannot read cross-section file " + path);
+    std::string line;
+    std::vector<std::vector<double>> rows;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::vector<double> row;
+        std::string tok;
+        while (ls >> tok) {
+            char* end = nullptr;
+            const double v = std::strtod(tok.c_str(), &end);
+            if (end == tok.c_str()) throw Error(MC_ERR_INPUT, "non-numeric token '" + tok + "' in " + path);
+            row.push_back(v);
+        }
+        if (!row.empty()) rows.push_back(row);
+    }
+    // line 1: A; lines 2-4: Watt parameters (xs_library/README.txt:1-7)
+    if (rows.size() < 5 || rows[0].size() != 1) throw Error(MC_ERR_INPUT, "bad header in " + path);
+    A = rows[0][0];
+    for (int k = 1; k <= 3; ++k)
+        if (rows[k].size() != 2) throw Error(MC_ERR_INPUT, "bad Watt parameters in " + path);
+    E.clear(); s.clear(); c.clear(); f.clear();
+    for (size_t k = 4; k < rows.size(); ++k) {
+        const auto& r = rows[k];
+        if (r.size() != 5 && r.size() != 6)
+            throw Error(MC_ERR_INPUT, "row " + std::to_string(k + 1) + " of " + path + " has " +
+                                          std::to_string(r.size()) + " columns (expected 5 or 6)");
+        E.push_back(r[0]);
+        s.push_back(r[1]);
+        c.push_back(r[2]);
+        f.push_back(r[3]);
+    }
+    for (size_t k = 1; k < E.size(); ++k)
+        if (!(E[k] > E[k - 1])) throw Error(MC_ERR_INPUT, "energy grid of " + path + " is not increasing");
+}
+
+namespace {
+
+template <class T>
+int find_name(const std::vector<std::string>& names, const T& n)
+{
+    for (size_t i = 0; i < names.size(); ++i)
+        if (names[i] == n) return (int)i;
+    return -1;
+}
+
+struct DistDouble {
+    int kind;        // MC_ENERGY_DELTA / MC_ENERGY_UNIFORM
+    double a, b;
+};
+struct DistPoint {
+    int kind;        // MC_DIR_DELTA / MC_DIR_ISOTROPIC
+    double x, y, z;
+};
+
+void need(bool ok, mc_status st, const std::string& msg)
+{
+    if (!ok) throw Error(st, msg);
+}
+
+}  // namespace
+
+void load_problem_xml(Problem& P, const std::string& xml, const std::string& xs_dir)
+{
+    std::vector<XmlNode> roots;
+    std::string err;
+    if (!xml_parse(xml, roots, err)) throw Error(MC_ERR_INPUT, "XML: " + err);
+    auto top = [&](const char* n) -> const XmlNode* {
+        for (const auto& r : roots)
+            if (r.name == n) return &r;
+        return nullptr;
+    };
+
+    // ---------------------------------------------------------- simulation
+    const XmlNode* sim = top("simulation");
+    need(sim && sim->child("description"), MC_ERR_INPUT, "missing <simulation><description>");
+    const XmlNode* descr = sim->child("description");
+    P.name = descr->value("name");
+    P.n_histories = (uint64_t)descr->as_double("samples");       // setup.cpp:50
+    if (descr->has("seed")) P.seed = (uint64_t)std::strtoull(descr->value("seed").c_str(), nullptr, 10);
+    need(P.seed > 0 && (P.seed & 1) == 1, MC_ERR_INPUT, "seed must be a positive odd integer");
+    need(!sim->child("ksearch"), MC_ERR_UNSUPPORTED, "k-eigenvalue search is out of scope (non-multiplying media only)");
+    need(!sim->child("tdmc"), MC_ERR_UNSUPPORTED, "time-dependent MC is out of scope");
+    need(!sim->child("trmm"), MC_ERR_UNSUPPORTED, "TRMM is out of scope");
+    if (const XmlNode* pc = top("population_control")) {
+        need(!pc->child("particle_comb"), MC_ERR_UNSUPPORTED, "particle comb is not supported");
+    }
+
+    // ------------------------------------------------------- distributions
+    std::map<std::string, DistDouble> ddist;
+    std::map<std::string, DistPoint> pdist;
+    if (const XmlNode* dists = top("distributions")) {              // setup.cpp:174-305
+        for (const auto& d : dists->children) {
+            const std::string name = d.value("name");
+            const std::string data = d.value("datatype");
+            if (data == "double") {
+                if (d.name == "delta") ddist[name] = {MC_ENERGY_DELTA, d.as_double("val"), 0.0};
+                else if (d.name == "uniform") ddist[name] = {MC_ENERGY_UNIFORM, d.as_double("a"), d.as_double("b")};
+                else if (d.name == "watt") throw Error(MC_ERR_UNSUPPORTED, "Watt fission spectrum is out of scope");
+                else throw Error(MC_ERR_INPUT, "unsupported double distribution " + d.name);
+            } else if (data == "point") {
+                if (d.name == "delta") pdist[name] = {MC_DIR_DELTA, d.as_double("x"), d.as_double("y"), d.as_double("z")};
+                else if (d.name == "isotropic") pdist[name] = {MC_DIR_ISOTROPIC, 0, 0, 0};
+                else if (d.name == "independentXYZ") throw Error(MC_ERR_UNSUPPORTED, "independentXYZ direction is not supported");
+                else throw Error(MC_ERR_INPUT, "unsupported point distribution " + d.name);
+            } else {
+                throw Error(MC_ERR_INPUT, "unsupported distribution datatype '" + data + "'");
+            }
+        }
+    }
+
+    // ------------------------------------------------------------ nuclides
+    if (const XmlNode* nucs = top("nuclides")) {                     // setup.cpp:310-467
+        for (const XmlNode* n : nucs->children_named("nuclide")) {
+            mc_nuclide_desc N{};
+            const std::string name = n->value("name");
+            if (n->has("ZAID")) {
+                const std::string path = xs_dir + "/" + n->value("ZAID") + ".txt";
+                std::vector<double> E, s, c, f;
+                load_xs_file(path, N.A, E, s, c, f);
+                for (double v : f)
+                    need(v == 0.0, MC_ERR_UNSUPPORTED, "nuclide " + name + " is fissile (non-multiplying media only)");
+                N.kind = MC_NUC_TABLE;
+                N.grid_begin = (int32_t)P.xs_energy.size();
+                N.grid_count = (int32_t)E.size();
+                for (size_t k = 0; k < E.size(); ++k) {
+                    P.xs_energy.push_back(E[k]);
+                    P.xs_scatter.push_back(s[k]);
+                    P.xs_capture.push_back(c[k]);
+                    P.xs_total.push_back(s[k] + c[k] + f[k]);       // setup.cpp:372
+                }
+            } else {
+                N.A = n->has("A") ? n->as_double("A") : MC_MASS_INFINITE;  // n_A = MAX_float, setup.cpp:313
+                if (N.A >= 3.0e38) N.A = MC_MASS_INFINITE;
+                N.kind = MC_NUC_CONSTANT;
+                for (const auto& r : n->children) {
+                    need(r.has("xs"), MC_ERR_INPUT, "unknown XS type in nuclide " + name);
+                    if (r.name == "capture") N.xs_capture = r.as_double("xs");
+                    else if (r.name == "scatter") N.xs_scatter = r.as_double("xs");
+                    else throw Error(MC_ERR_INPUT, "user-defined nuclide " + name + " supports capture and scatter only");
+                }
+                N.xs_total = N.xs_scatter + N.xs_capture + 0.0;
+            }
+            need(N.A > 0.0, MC_ERR_INPUT, "nuclide " + name + " has non-positive mass");
+            P.nuclides.push_back(N);
+            P.nuclide_names.push_back(name);
+        }
+    }
+
+    // ----------------------------------------------------------- materials
+    if (const XmlNode* mats = top("materials")) {                    // setup.cpp:474-492
+        for (const XmlNode* m : mats->children_named("material")) {
+            mc_material_desc M{};
+            M.nuc_begin = (int32_t)P.mat_nuclide.size();
+            for (const XmlNode* n : m->children_named("nuclide")) {
+                const int k = find_name(P.nuclide_names, n->value("name"));
+                need(k >= 0, MC_ERR_INPUT, "unknown nuclide '" + n->value("name") + "' in material " + m->value("name"));
+                P.mat_nuclide.push_back(k);
+                P.mat_density.push_back(n->as_double("density"));
+            }
+            M.nuc_count = (int32_t)P.mat_nuclide.size() - M.nuc_begin;
+            P.materials.push_back(M);
+            P.material_names.push_back(m->value("name"));
+        }
+    }
+
+    // ------------------------------------------------------------ surfaces
+    if (const XmlNode* surfs = top("surfaces")) {                    // setup.cpp:499-584
+        for (const auto& s : surfs->children) {
+            mc_surface_desc S{};
+            std::string bc = "transmission";
+            if (s.has("bc")) bc = s.value("bc");
+            if (bc == "transmission") S.bc = MC_BC_TRANSMISSION;
+            else if (bc == "reflective") S.bc = MC_BC_REFLECTIVE;
+            else if (bc == "vacuum") S.bc = MC_BC_VACUUM;
+            else throw Error(MC_ERR_INPUT, "unknown boundary condition '" + bc + "'");
+            if (s.name == "plane_x") { S.type = MC_SURF_PLANE_X; S.c[0] = s.as_double("x"); }
+            else if (s.name == "plane_y") { S.type = MC_SURF_PLANE_Y; S.c[0] = s.as_double("y"); }
+            else if (s.name == "plane_z") { S.type = MC_SURF_PLANE_Z; S.c[0] = s.as_double("z"); }
+            else if (s.name == "plane") {
+                S.type = MC_SURF_PLANE;
+                const double a = s.as_double("a"), b = s.as_double("b"), c = s.as_double("c");
+                S.c[0] = a; S.c[1] = b; S.c[2] = c; S.c[3] = s.as_double("d");
+                const double L = 2.0 / (a * a + b * b + c * c);       // Geometry.cpp:19-22
+                S.c[4] = L * a; S.c[5] = L * b; S.c[6] = L * c;
+            } else if (s.name == "sphere") {
+                S.type = MC_SURF_SPHERE;
+                S.c[0] = s.as_double("x"); S.c[1] = s.as_double("y"); S.c[2] = s.as_double("z");
+                S.c[3] = s.as_double("r"); S.c[4] = S.c[3] * S.c[3];
+            } else if (s.name == "cylinder_x") {
+                S.type = MC_SURF_CYL_X;
+                S.c[0] = s.as_double("y"); S.c[1] = s.as_double("z");
+                S.c[2] = s.as_double("r"); S.c[4] = S.c[2] * S.c[2];
+            } else if (s.name == "cylinder_z") {
+                S.type = MC_SURF_CYL_Z;
+                S.c[0] = s.as_double("x"); S.c[1] = s.as_double("y");
+                S.c[2] = s.as_double("r"); S.c[4] = S.c[2] * S.c[2];
+            } else {
+                throw Error(MC_ERR_INPUT, "unknown surface type " + s.name);
+            }
+            P.surfaces.push_back(S);
+            P.surface_names.push_back(s.value("name"));
+        }
+    }
+
+    // --------------------------------------------------------------- cells
+    if (const XmlNode* cells = top("cells")) {                       // setup.cpp:591-629
+        for (const XmlNode* c : cells->children_named("cell")) {
+            mc_cell_desc C{};
+            std::string name = "Cell " + std::to_string(P.cells.size() + 1);
+            if (c->has("name")) name = c->value("name");
+            C.importance = c->has("importance") ? c->as_double("importance") : 1.0;
+            C.material = -1;
+            if (c->has("material")) {
+                C.material = find_name(P.material_names, c->value("material"));
+                need(C.material >= 0, MC_ERR_INPUT, "unknown material in cell " + name);
+            }
+            C.surf_begin = (int32_t)P.cell_surf.size();
+            for (const XmlNode* s : c->children_named("surface")) {
+                const int k = find_name(P.surface_names, s->value("name"));
+                need(k >= 0, MC_ERR_INPUT, "unknown surface '" + s->value("name") + "' in cell " + name);
+                P.cell_surf.push_back(k);
+                P.cell_sense.push_back(s->as_int("sense"));
+            }
+            C.surf_count = (int32_t)P.cell_surf.size() - C.surf_begin;
+            need(C.importance >= 0.0, MC_ERR_INPUT, "negative importance in cell " + name);
+            P.cells.push_back(C);
+            P.cell_names.push_back(name);
+        }
+    }
+
+    // ---------------------------------------------------------- estimators
+    // Attachments are collected per geometry in deck order (Surface::estimators,
+    // Cell::estimators_TL / estimators_C), then laid out in CSR form.
+    std::vector<std::vector<std::pair<int, int>>> surf_att(P.surfaces.size()), tl_att(P.cells.size()),
+        col_att(P.cells.size());
+    if (const XmlNode* ests = top("estimators")) {                  // setup.cpp:637-805
+        for (const XmlNode* e : ests->children_named("estimator")) {
+            mc_estimator_desc E{};
+            const int eid = (int)P.estimators.size();
+            std::string name = "Estimator #" + std::to_string(eid + 1);
+            if (e->has("name")) name = e->value("name");
+            std::string type = "TL";
+            if (e->has("type")) type = e->value("type");
+            int kern;
+            if (type == "TL") kern = MC_KERNEL_TRACK_LENGTH;
+            else if (type == "C") kern = MC_KERNEL_COLLISION;
+            else throw Error(MC_ERR_INPUT, "unsupported score type in estimator " + name);
+            need(e->has("scores"), MC_ERR_INPUT, "there is no score in estimator " + name);
+            std::istringstream iss(e->value("scores"));
+            std::vector<std::string> snames;
+            for (std::string s; iss >> s;) {
+                need(E.n_scores < MC_MAX_SCORES, MC_ERR_CAPACITY, "too many scores in estimator " + name);
+                int sc;
+                if (s == "flux") sc = MC_SCORE_FLUX;
+                else if (s == "absorption") sc = MC_SCORE_ABSORPTION;
+                else if (s == "scatter") sc = MC_SCORE_SCATTER;
+                else if (s == "capture") sc = MC_SCORE_CAPTURE;
+                else if (s == "total") sc = MC_SCORE_TOTAL;
+                else if (s == "cross") { kern = MC_KERNEL_NEUTRON; sc = MC_SCORE_FLUX; }   // setup.cpp:688-690
+                else if (s == "fission" || s == "nu-fission")
+                    throw Error(MC_ERR_UNSUPPORTED, "fission scores are out of scope (non-multiplying media)");
+                else throw Error(MC_ERR_INPUT, "unsupported score type " + s + " in estimator " + name);
+                E.score[E.n_scores] = sc;
+                E.score_kernel[E.n_scores] = kern;
+                E.n_scores++;
+                snames.push_back(s);
+            }
+            need(E.n_scores > 0, MC_ERR_INPUT, "there is no score in estimator " + name);
+            need(!e->child("tdmc"), MC_ERR_UNSUPPORTED, "TDMC filters are out of scope");
+            std::vector<double> grid;
+            const auto surfs = e->children_named("surface");
+            const auto cells = e->children_named("cell");
+            need(!(surfs.size() && cells.size()), MC_ERR_UNSUPPORTED,
+                 "estimator " + name + " attached to both surfaces and cells");
+            std::vector<std::pair<int, bool>> attached;   // (geometry, is_surface)
+            for (const XmlNode* s : surfs) {
+                const int k = find_name(P.surface_names, s->value("name"));
+                need(k >= 0, MC_ERR_INPUT, "unknown surface label " + s->value("name") + " in estimator " + name);
+                attached.emplace_back(k, true);
+                grid.push_back((double)k);
+            }
+            for (const XmlNode* c : cells) {
+                const int k = find_name(P.cell_names, c->value("name"));
+                need(k >= 0, MC_ERR_INPUT, "unknown cell label " + c->value("name") + " in estimator " + name);
+                attached.emplace_back(k, false);
+                grid.push_back((double)k);
+            }
+            need(!attached.empty(), MC_ERR_INPUT, "estimator " + name + " needs to be attached somewhere");
+            E.attach_kind = surfs.empty() ? MC_ATTACH_CELL : MC_ATTACH_SURFACE;
+            E.n_attach = (int32_t)grid.size();
+            for (const auto& a : attached) {
+                // FilterSurface/FilterCell::idx_l: binary_search(ID, grid) + 1 (Estimator.cpp:140,156)
+                const int fi = binary_search((double)a.first, grid.data(), (int)grid.size()) + 1;
+                if (a.second) surf_att[a.first].emplace_back(eid, fi);
+                else if (kern == MC_KERNEL_COLLISION || type == "C") col_att[a.first].emplace_back(eid, fi);
+                else tl_att[a.first].emplace_back(eid, fi);
+            }
+            std::vector<double> eg;
+            for (const XmlNode* f : e->children_named("filter")) {
+                const std::string ft = f->value("type");
+                std::vector<double> fg;
+                if (f->has("grid")) {
+                    std::istringstream g(f->value("grid"));
+                    for (double v; g >> v;) fg.push_back(v);
+                } else if (f->has("grid_linear")) {                    // setup.cpp:753-763
+                    std::istringstream g(f->value("grid_linear"));
+                    double a, step, b;
+                    g >> a >> step >> b;
+                    need(step > 0, MC_ERR_INPUT, "grid_linear step must be positive");
+                    fg.push_back(a);
+                    while (fg.back() < b) fg.push_back(fg.back() + step);
+                    fg.pop_back();
+                    fg.push_back(b);
+                } else if (f->has("grid_lethargy")) {                  // setup.cpp:764-776
+                    std::istringstream g(f->value("grid_lethargy"));
+                    double a, b, N;
+                    g >> a >> b >> N;
+                    need(N >= 1 && a > 0 && b > a, MC_ERR_INPUT, "bad grid_lethargy in estimator " + name);
+                    const double step = std::log(b / a) / N;
+                    fg.push_back(0.0);
+                    while ((double)fg.size() != N + 1) fg.push_back(fg.back() + step);
+                    std::reverse(fg.begin(), fg.end());
+                    for (auto& v : fg) v = b * std::exp(-v);
+                } else {
+                    throw Error(MC_ERR_INPUT, "need filter grid for estimator " + name);
+                }
+                if (ft == "energy") {
+                    need(eg.empty(), MC_ERR_UNSUPPORTED, "more than one energy filter in estimator " + name);
+                    need(fg.size() >= 2, MC_ERR_INPUT, "energy filter needs at least two edges");
+                    eg = fg;
+                } else if (ft == "time") {
+                    throw Error(MC_ERR_UNSUPPORTED, "time filters are out of scope (no time-dependent tallies)");
+                } else {
+                    throw Error(MC_ERR_INPUT, "unknown filter type for estimator " + name);
+                }
+            }
+            E.egrid_begin = (int32_t)P.egrid.size();
+            E.egrid_count = (int32_t)eg.size();
+            P.egrid.insert(P.egrid.end(), eg.begin(), eg.end());
+            const int NE = eg.empty() ? 1 : (int)eg.size() - 1;
+            E.bin_offset = P.n_bins;
+            E.n_bins = E.n_scores * E.n_attach * NE;
+            P.n_bins += E.n_bins;
+            P.estimators.push_back(E);
+            P.estimator_names.push_back(name);
+            P.estimator_scores.push_back(snames);
+            P.estimator_attach_ids.push_back(grid);
+            P.estimator_egrid.push_back(eg);
+        }
+    }
+    for (size_t s = 0; s < P.surfaces.size(); ++s) {
+        P.surfaces[s].est_begin = (int32_t)P.att_estimator.size();
+        for (const auto& a : surf_att[s]) { P.att_estimator.push_back(a.first); P.att_index.push_back(a.second); }
+        P.surfaces[s].est_count = (int32_t)P.att_estimator.size() - P.surfaces[s].est_begin;
+    }
+    for (size_t c = 0; c < P.cells.size(); ++c) {
+        P.cells[c].tl_begin = (int32_t)P.att_estimator.size();
+        for (const auto& a : tl_att[c]) { P.att_estimator.push_back(a.first); P.att_index.push_back(a.second); }
+        P.cells[c].tl_count = (int32_t)P.att_estimator.size() - P.cells[c].tl_begin;
+        P.cells[c].col_begin = (int32_t)P.att_estimator.size();
+        for (const auto& a : col_att[c]) { P.att_estimator.push_back(a.first); P.att_index.push_back(a.second); }
+        P.cells[c].col_count = (int32_t)P.att_estimator.size() - P.cells[c].col_begin;
+    }
+
+    // ---------------------------------------------------------- parameters
+    if (const XmlNode* prm = top("parameters")) {
+        std::vector<int> clone_of_cell(P.cells.size(), -1);
+        for (const XmlNode* q : prm->children_named("parameter")) {
+            const int c = find_name(P.cell_names, q->value("cell"));
+            need(c >= 0, MC_ERR_INPUT, "unknown cell '" + q->value("cell") + "' in parameter");
+            need(P.cells[c].material >= 0, MC_ERR_INPUT, "parameter on void cell " + P.cell_names[c]);
+            if (clone_of_cell[c] < 0) {      // per-cell material instance
+                const mc_material_desc base = P.materials[P.cells[c].material];
+                mc_material_desc M{};
+                M.nuc_begin = (int32_t)P.mat_nuclide.size();
+                M.nuc_count = base.nuc_count;
+                for (int k = 0; k < base.nuc_count; ++k) {
+                    P.mat_nuclide.push_back(P.mat_nuclide[base.nuc_begin + k]);
+                    P.mat_density.push_back(P.mat_density[base.nuc_begin + k]);
+                }
+                P.materials.push_back(M);
+                P.material_names.push_back(P.material_names[P.cells[c].material] + "@" + P.cell_names[c]);
+                clone_of_cell[c] = (int)P.materials.size() - 1;
+                P.cells[c].material = clone_of_cell[c];
+            }
+            const int m = clone_of_cell[c];
+            int slot = -1;
+            for (int k = 0; k < P.materials[m].nuc_count; ++k)
+                if (P.nuclide_names[P.mat_nuclide[P.materials[m].nuc_begin + k]] == q->value("nuclide")) { slot = k; break; }
+            need(slot >= 0, MC_ERR_INPUT, "nuclide '" + q->value("nuclide") + "' not in the material of cell " + P.cell_names[c]);
+            const std::string r = q->has("reaction") ? q->value("reaction") : std::string("total");
+            mc_param_desc D{};
+            D.cell = c;
+            D.material = m;
+            D.nuc_slot = slot;
+            D.reactions = r == "scatter" ? MC_REACTION_SCATTER : r == "capture" ? MC_REACTION_CAPTURE
+                        : r == "total" ? (MC_REACTION_SCATTER | MC_REACTION_CAPTURE) : 0;
+            need(D.reactions != 0, MC_ERR_INPUT, "parameter reaction must be scatter, capture or total");
+            P.params.push_back(D);
+        }
+    }
+    for (size_t m = 0; m < P.materials.size(); ++m) {
+        P.materials[m].param_begin = (int32_t)P.mat_param.size();
+        for (size_t k = 0; k < P.params.size(); ++k)
+            if (P.params[k].material == (int)m) P.mat_param.push_back((int32_t)k);
+        P.materials[m].param_count = (int32_t)P.mat_param.size() - P.materials[m].param_begin;
+    }
+
+    // ------------------------------------------------------------- sources
+    if (const XmlNode* srcs = top("sources")) {                     // setup.cpp:1020-1066
+        for (const auto& s : srcs->children) {
+            mc_source_desc S{};
+            S.probability = s.has("probability") ? s.as_double("probability") : 1.0;
+            S.dir_kind = MC_DIR_ISOTROPIC;                           // setup.cpp:1027-1030
+            S.energy_kind = MC_ENERGY_DELTA;
+            S.energy[0] = 2e6;
+            if (s.has("direction")) {
+                auto it = pdist.find(s.value("direction"));
+                need(it != pdist.end(), MC_ERR_INPUT, "unknown direction distribution in source");
+                S.dir_kind = it->second.kind;
+                S.dir[0] = it->second.x; S.dir[1] = it->second.y; S.dir[2] = it->second.z;
+            }
+            if (s.has("energy")) {
+                auto it = ddist.find(s.value("energy"));
+                need(it != ddist.end(), MC_ERR_INPUT, "unknown energy distribution in source");
+                S.energy_kind = it->second.kind;
+                S.energy[0] = it->second.a;
+                S.energy[1] = it->second.b;
+            }
+            need(s.name == "point", s.name == "disk_z" ? MC_ERR_UNSUPPORTED : MC_ERR_INPUT,
+                 "unknown source type: " + s.name);
+            S.pos[0] = s.as_double("x"); S.pos[1] = s.as_double("y"); S.pos[2] = s.as_double("z");
+            S.cell = P.search_cell(S.pos);
+            need(S.cell >= 0, MC_ERR_LOST_PARTICLE, "source point is outside every cell");
+            P.sources.push_back(S);
+        }
+    }
+    need(!P.sources.empty(), MC_ERR_INPUT, "the deck defines no source");
+    // SourceBank::set_up, Source.cpp:65-72: equal probabilities by repeated addition
+    P.source_cdf.assign(P.sources.size() + 1, 0.0);
+    const double dp = 1.0 / (double)P.sources.size();
+    for (size_t i = 1; i < P.source_cdf.size(); ++i) P.source_cdf[i] = P.source_cdf[i - 1] + dp;
+    need(P.n_histories > 1, MC_ERR_INPUT, "samples must be at least 2");
+    P.finalize();
+}
+
+}  // namespace mc
