@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -504,6 +505,8 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             graph_trace[gi] = (const void*)A.trace;
         }
     }
+    const auto t0 = std::chrono::steady_clock::now();
+    const double max_s = getenv("MC_MAX_SECONDS") ? atof(getenv("MC_MAX_SECONDS")) : 3600.0;
     for (uint64_t chunk = 0;; ++chunk) {
         if (cfg.use_graph) HIP_CHECK(hipGraphLaunch(graph[gi], stream));
         else
@@ -512,7 +515,10 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
         HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         if (h_ctr->nA == 0) break;
-        if (chunk > (1ull << 40)) throw Error(MC_ERR_DEVICE, "event loop did not terminate");
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (el > max_s)
+            throw Error(MC_ERR_CAPACITY, "event loop exceeded " + std::to_string(max_s) +
+                                         " s (MC_MAX_SECONDS); live particles: " + std::to_string(h_ctr->nA));
     }
 }
 

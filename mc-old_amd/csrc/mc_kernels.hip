@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(1024) k_refill(KArgs A)
             const DProb PR = A.prob[p];
             uint64_t rng = history_seed(PR.seed, hid, A.jump);
             const double xi = urand(rng);
-            const int si = binary_search(xi, A.src_cdf + A.cdf_off[p], PR.src_count + 1);
+            int si = binary_search(xi, A.src_cdf + A.cdf_off[p], PR.src_count + 1);
+            si = si < 0 ? 0 : (si >= PR.src_count ? PR.src_count - 1 : si);   // cdf[n] may round below 1
             const mc_source_desc& S = A.src[PR.src_begin + si];
             double ux, uy, uz;
             if (S.dir_kind == MC_DIR_DELTA) {

@@ -372,6 +372,8 @@ void load_problem_xml(Problem& P, const std::string& xml, const std::string& xs_
             for (const auto& a : attached) {
                 // FilterSurface/FilterCell::idx_l: binary_search(ID, grid) + 1 (Estimator.cpp:140,156)
                 const int fi = binary_search((double)a.first, grid.data(), (int)grid.size()) + 1;
+                need(fi >= 0 && fi < (int)grid.size(), MC_ERR_INPUT,
+                     "attachment index out of range in estimator " + name);
                 if (a.second) surf_att[a.first].emplace_back(eid, fi);
                 else if (kern == MC_KERNEL_COLLISION || type == "C") col_att[a.first].emplace_back(eid, fi);
                 else tl_att[a.first].emplace_back(eid, fi);

@@ -729,7 +729,8 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
         P.rng = mco_init_particle(d->seed, key_offset + h);
         /* SourceBank::get_source, Source.cpp:42-45, then SourcePoint, :20-24 */
         const double xi = mco_urand(&P.rng);
-        const int si = mco_binary_search(xi, d->source_cdf, d->n_sources + 1);
+        int si = mco_binary_search(xi, d->source_cdf, d->n_sources + 1);
+        si = si < 0 ? 0 : (si >= d->n_sources ? d->n_sources - 1 : si);   /* cdf[n] may round below 1 */
         const mc_source_desc* src = &d->sources[si];
         P.p[0] = src->pos[0]; P.p[1] = src->pos[1]; P.p[2] = src->pos[2];
         if (src->dir_kind == MC_DIR_DELTA) {

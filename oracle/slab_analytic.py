@@ -108,7 +108,7 @@ def _extrapolate(hs, vals):
 
 def slab(L: float, c: float, n_bins: int = 100, nodes_per_bin: int = 4):
     """T, R and the n_bins track-length flux bins (three-grid extrapolation)."""
-    n = max(n_bins * nodes_per_bin, int(np.ceil(L * 16)))
+    n = max(n_bins * nodes_per_bin, int(np.ceil(L * 32)))
     n += (-n) % n_bins
     edges = np.linspace(0.0, L, n_bins + 1)
     hs, TR, FL = [], [], []

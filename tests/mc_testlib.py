@@ -1,0 +1,149 @@
+"""Test helpers: the oracle (CPU restatement) through ctypes, and fixtures
+shared by the CPU and GPU tests.  The oracle is the checker only."""
+from __future__ import annotations
+
+import ctypes as C
+import importlib
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "libmc_oracle.so")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_kernels.so")
+
+mc = importlib.import_module("mc-old_amd")
+decks = importlib.import_module("mc-old_amd.decks")
+
+
+def _oracle():
+    o = C.CDLL(ORACLE_LIB)
+    o.mco_lcg_next.restype = C.c_uint64
+    o.mco_lcg_next.argtypes = [C.c_uint64]
+    o.mco_urand.restype = C.c_double
+    o.mco_urand.argtypes = [C.POINTER(C.c_uint64)]
+    o.mco_skip_ahead.restype = C.c_uint64
+    o.mco_skip_ahead.argtypes = [C.c_uint64, C.c_int64]
+    o.mco_init_particle.restype = C.c_uint64
+    o.mco_init_particle.argtypes = [C.c_uint64, C.c_uint64]
+    o.mco_binary_search.restype = C.c_int
+    o.mco_binary_search.argtypes = [C.c_double, C.POINTER(C.c_double), C.c_int]
+    o.mco_interpolate.restype = C.c_double
+    o.mco_interpolate.argtypes = [C.c_double] * 5
+    o.mco_geometry_quad.restype = C.c_double
+    o.mco_geometry_quad.argtypes = [C.c_double] * 3
+    o.mco_scatter_direction.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_double)]
+    o.mco_log.restype = C.c_double
+    o.mco_log.argtypes = [C.c_double]
+    o.mco_sincos_2pi.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    o.mco_surface_eval.restype = C.c_double
+    o.mco_surface_eval.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    o.mco_surface_distance.restype = C.c_double
+    o.mco_surface_distance.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    o.mco_surface_reflect.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    o.mco_nuclide_xs.restype = C.c_double
+    o.mco_nuclide_xs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+    o.mco_material_xs.restype = C.c_double
+    o.mco_material_xs.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double]
+    o.mco_run.restype = C.c_int
+    o.mco_run.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p,
+                          C.POINTER(C.c_uint64), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+    return o
+
+
+oracle = _oracle()
+
+
+def ref_lib():
+    """The reference's own Random/Algorithm/XSec compiled into oracle/_ref (or None)."""
+    if not os.path.exists(REF_LIB):
+        return None
+    r = C.CDLL(REF_LIB)
+    r.ref_rn_init_problem.argtypes = [C.c_uint64]
+    r.ref_rn_init_particle.argtypes = [C.c_uint64]
+    r.ref_urand.restype = C.c_double
+    r.ref_skip_ahead.restype = C.c_uint64
+    r.ref_skip_ahead.argtypes = [C.c_uint64, C.c_int64]
+    r.ref_binary_search.restype = C.c_int
+    r.ref_binary_search.argtypes = [C.c_double, C.POINTER(C.c_double), C.c_int]
+    r.ref_interpolate.restype = C.c_double
+    r.ref_interpolate.argtypes = [C.c_double] * 5
+    r.ref_geometry_quad.restype = C.c_double
+    r.ref_geometry_quad.argtypes = [C.c_double] * 3
+    r.ref_scatter_direction.argtypes = [C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
+    r.ref_exponential_sample.restype = C.c_double
+    r.ref_exponential_sample.argtypes = [C.c_double]
+    r.ref_xstable.restype = C.c_double
+    r.ref_xstable.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_double]
+    return r
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleResult:
+    def __init__(self, n, nb, npar):
+        self.n = n
+        self.sum = np.zeros(max(1, nb))
+        self.sq = np.zeros(max(1, nb))
+        self.gsum = np.zeros(max(1, npar))
+        self.gsq = np.zeros(max(1, npar))
+        self.flights = 0
+        self.trace = None
+
+    @property
+    def mean(self):
+        return self.sum / self.n
+
+    @property
+    def uncer(self):
+        m = self.mean
+        return np.sqrt((self.sq / self.n - m * m) / (self.n - 1.0))
+
+    @property
+    def grad(self):
+        return self.gsum / self.n
+
+    @property
+    def grad_sigma(self):
+        m = self.grad
+        return np.sqrt((self.gsq / self.n - m * m) / (self.n - 1.0))
+
+
+def oracle_run(problem, n: int, key_offset: int = 0, theta=None, trace: int = 0, grad_out=None) -> OracleResult:
+    """Runs n histories of `problem` on the CPU restatement."""
+    r = OracleResult(n, problem.n_bins, problem.n_params)
+    fl = C.c_uint64(0)
+    tr = None
+    if trace:
+        tr = np.zeros(trace, dtype=mc.TRACE_DTYPE)
+    th = None if theta is None else np.ascontiguousarray(theta, dtype=np.float64)
+    go = None if grad_out is None else np.ascontiguousarray(grad_out, dtype=np.float64)
+    st = oracle.mco_run(problem.desc_ptr, None if th is None else th.ctypes.data, key_offset, n,
+                        r.sum.ctypes.data, r.sq.ctypes.data, C.byref(fl),
+                        None if tr is None else tr.ctypes.data, trace,
+                        None if go is None else go.ctypes.data,
+                        r.gsum.ctypes.data if go is not None else None,
+                        r.gsq.ctypes.data if go is not None else None)
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    r.flights = fl.value
+    r.trace = tr
+    r.sum = r.sum[:problem.n_bins]
+    r.sq = r.sq[:problem.n_bins]
+    r.gsum = r.gsum[:problem.n_params]
+    r.gsq = r.gsq[:problem.n_params]
+    return r
+
+
+_SHIELD_XS = os.path.join(ROOT, "tests", "_xs_cache")
+
+
+def shield_xs_dir() -> str:
+    """Directory holding the synthetic shield nuclides (written on first use)."""
+    marker = os.path.join(_SHIELD_XS, "926056.txt")
+    if not os.path.exists(marker):
+        decks.write_shield_nuclides(_SHIELD_XS)
+    return _SHIELD_XS
