@@ -1,0 +1,241 @@
+#!/usr/bin/env python
+"""Benchmark: particle histories/s of the MI355X event-based engine.
+
+Headline workload: the continuous-energy three-layer shield (water-like
+20 cm / iron-like 10 cm / concrete-like 30 cm over four synthetic nuclides,
+16384-point log grids, isotropic 2 MeV point source), 1e8 histories per step,
+with cross-section tables and problem data resident in HBM before timing.
+
+Also reported on the same JSON line:
+  batched_vs_plain  the 4096-slab sweep in one batched launch sequence vs one
+                    problem per launch sequence (plain timed on a seeded subset)
+  bwd_fwd           likelihood-ratio backward cost / forward cost, 24-parameter shield
+  xs_lookup         the standalone lookup kernel against HBM peak
+  roofline          the dominant event-loop kernel (algorithmic bytes per particle
+                    event, device time from per-kernel HIP events)
+  tally_check       GPU tallies vs the CPU oracle sample (combined-sigma z scores)
+  cpu_baseline      the oracle (C restatement, 1 core) on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD public spec
+
+# Algorithmic HBM bytes per particle event for each event kernel (particle
+# state read + written, queue entries); tables and geometry are cache-resident
+# and not counted.
+ALGO_BYTES = {
+    "k_xs_lookup": 24,       # queue 4 + cell 4 + E 8 in, Sigma_t 8 out
+    "k_flight": 137,         # queue 4, cell 4, pos 24, dir 24, rng 8, E 8, Sigma_t 8, prob 4, count 4, wgt 8 in;
+                             # pos 24, rng 8, count 4, surface 4, status 1 out
+    "k_event_scatter": 9,    # queue 4 + status 1 in, queue entry 4 out
+    "k_surface": 157,        # queue 4, surface 4, prob 4, pos 24, dir 24, wgt 8, rng 8, E 8, cell 4 in;
+                             # pos 24, dir 24, wgt 8, rng 8, cell 4, status 1 out
+    "k_collision": 109,      # queue 4, cell 4, prob 4, wgt 8, rng 8, E 8, dir 24 in; dir 24, E 8, wgt 8, rng 8, status 1 out
+    "k_refill_count": 1,     # status byte per slot
+    "k_refill_scatter": 5,   # status byte per slot + queue entry per live slot
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def shield_xs_dir():
+    d = os.path.join(ROOT, "_bench_xs")
+    decks = importlib.import_module("mc-old_amd.decks")
+    if not os.path.exists(os.path.join(d, "926056.txt")):
+        decks.write_shield_nuclides(d)
+    return d
+
+
+def cpu_baseline(mc, decks, xs, n_sample):
+    from mc_testlib import oracle_run
+    p = mc.Problem.from_string(decks.shield_deck(n_sample), xs)
+    t = time.perf_counter()
+    r = oracle_run(p, n_sample, key_offset=10 ** 12)
+    dt = time.perf_counter() - t
+    return r, dt
+
+
+def zscores(gm, gu, om, ou, min_rel=0.2):
+    """Combined-sigma z scores over bins the smaller CPU sample resolves
+    (relative error <= min_rel, i.e. roughly 25 or more contributions)."""
+    s = np.sqrt(gu ** 2 + ou ** 2)
+    ok = (s > 0) & (om > 0) & (ou <= min_rel * np.abs(om))
+    return (gm[ok] - om[ok]) / s[ok]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--histories", type=float, default=1e8)
+    ap.add_argument("--cpu-sample", type=int, default=100000)
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+              "(histories shard with no communication; see the sharding design)", file=sys.stderr)
+        sys.exit(2)
+
+    mc = importlib.import_module("mc-old_amd")
+    decks = importlib.import_module("mc-old_amd.decks")
+    xs = shield_xs_dir()
+    H = int(args.histories)
+
+    # ---------------------------------------------------------- headline
+    prob = mc.Problem.from_string(decks.shield_deck(H), xs)
+    G = not args.no_graph
+    eng = mc.Engine(prob, use_graph=G)
+    for _ in range(args.warmup):
+        eng.forward(n_histories=H)
+    steps = []
+    res = None
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        res = eng.forward(n_histories=H)
+        steps.append(time.perf_counter() - t)
+    elapsed = time.perf_counter() - t_all
+    value = H * args.steps / elapsed
+    log(f"shield: {H:.3g} histories/step, {np.mean(steps):.3f} s/step, {value:.4g} hist/s, "
+        f"{res.flights / np.mean(steps):.4g} flights/s, T = {res.mean[0]:.5g} +- {res.uncer[0]:.2g}")
+    out = {
+        "metric": "particle histories/s at <=3 sigma tally agreement, plain vs batched; bwd/fwd cost",
+        "value": value,
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded synthetic nuclides, deck generated in-process)",
+        "config": {"workload": "continuous-energy 3-layer shield (water 20 / iron 10 / concrete 30 cm), "
+                               "4 synthetic nuclides A=1,12,16,56 on 16384-point log grids, "
+                               "isotropic 2 MeV point source, transmission + 2x100-bin energy leakage",
+                   "histories_per_step": H, "flights_per_step": int(res.flights),
+                   "events_per_second": res.flights / np.mean(steps),
+                   "bank_capacity": int(eng.info.bank_capacity)},
+    }
+
+    # ------------------------------------------------- tally agreement + CPU
+    n_cpu = args.cpu_sample
+    r_cpu, dt_cpu = cpu_baseline(mc, decks, xs, n_cpu)
+    z = zscores(res.mean, res.uncer, r_cpu.mean, r_cpu.uncer)
+    out["tally_check"] = {"reference": f"oracle, {n_cpu} histories (disjoint keys)", "bins": int(z.size),
+                          "max_abs_z": float(np.abs(z).max()) if z.size else 0.0,
+                          "frac_abs_z_gt_3": float(np.mean(np.abs(z) > 3)) if z.size else 0.0}
+    out["cpu_baseline"] = {"value": n_cpu / dt_cpu, "unit": "histories/s", "cores": 1, "kind": "port",
+                           "sample": f"{n_cpu} histories of the same shield deck (oracle C restatement, "
+                                     f"{dt_cpu:.1f} s, {platform.processor() or platform.machine()}, "
+                                     f"nproc={os.cpu_count()})"}
+    log(f"cpu oracle: {n_cpu / dt_cpu:.4g} hist/s; tally z max {out['tally_check']['max_abs_z']:.2f}")
+
+    if not args.quick:
+        # -------------------------------------------- per-kernel roofline
+        peng = mc.Engine(prob, profile=True)
+        nprof = max(2, H // 10)
+        peng.forward(n_histories=nprof)
+        ks = peng.kernel_stats()
+        table = {}
+        for k, s in ks.items():
+            if s["launches"] == 0 or s["seconds"] == 0:
+                continue
+            b = ALGO_BYTES.get(k)
+            gbs = b * s["items"] / s["seconds"] / 1e9 if b else None
+            table[k] = {"seconds": s["seconds"], "launches": s["launches"], "items": s["items"],
+                        "avg_us": 1e6 * s["seconds"] / s["launches"], "algo_GBs": gbs}
+        dom = max((k for k in table if k in ALGO_BYTES), key=lambda k: table[k]["seconds"])
+        d = table[dom]
+        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["algo_GBs"], "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                           "bytes_per_event": ALGO_BYTES[dom], "avg_launch_us": d["avg_us"],
+                           "events_per_launch": d["items"] / d["launches"]}
+        out["kernels"] = table
+        log("kernels:", json.dumps(table))
+
+        # ------------------------------------------ standalone XS lookup
+        n_xs = 1 << 26
+        sec, _ = eng.xs_bench(n_xs, iters=20)
+        gbs = 20 * n_xs / sec / 1e9
+        out["xs_lookup"] = {"lookups": n_xs, "seconds_per_launch": sec, "lookups_per_s": n_xs / sec,
+                            "algo_bytes_per_lookup": 20, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        log(f"xs lookup: {n_xs / sec:.4g} lookups/s, {gbs:.1f} GB/s ({gbs / HBM_PEAK_GBS:.1%})")
+
+        # -------------------------------------- batched vs plain sweep
+        L, c, N = decks.batched_slabs(4096)
+        probs = [mc.Problem.from_string(decks.slab_deck(float(l), float(cc), 0, int(k)))
+                 for l, cc, k in zip(L, c, N)]
+        beng = mc.Engine(probs, use_graph=G)
+        beng.forward(n_histories=np.maximum(2, N // 1000))   # warm-up (graph capture)
+        t = time.perf_counter()
+        br = beng.forward()
+        tb = time.perf_counter() - t
+        rate_b = float(N.sum()) / tb
+        rng = np.random.default_rng(11)
+        sub = np.sort(rng.choice(4096, 64, replace=False))
+        peng2 = mc.Engine([probs[i] for i in sub], use_graph=G)
+        peng2.forward(n_histories=np.maximum(2, N[sub] // 1000), mode=mc.MC_RUN_PLAIN)
+        t = time.perf_counter()
+        peng2.forward(mode=mc.MC_RUN_PLAIN)
+        tp = time.perf_counter() - t
+        rate_p = float(N[sub].sum()) / tp
+        out["batched_vs_plain"] = {"problems": 4096, "histories": int(N.sum()), "batched_seconds": tb,
+                                   "batched_hist_per_s": rate_b, "plain_subset": int(sub.size),
+                                   "plain_seconds": tp, "plain_hist_per_s": rate_p, "speedup": rate_b / rate_p,
+                                   "flights": int(br.flights)}
+        golden = os.path.join(ROOT, "tests", "golden", "batched_slabs_analytic.npz")
+        if os.path.exists(golden):
+            g = np.load(golden)
+            ref = np.stack([g["T"], g["R"]], axis=1).ravel()
+            zz = (br.mean - ref) / br.uncer
+            out["batched_vs_plain"]["tally_check"] = {"reference": "deterministic slab solution",
+                                                      "max_abs_z": float(np.abs(zz).max()),
+                                                      "frac_abs_z_gt_3": float(np.mean(np.abs(zz) > 3))}
+        log(f"sweep: batched {rate_b:.4g} hist/s ({tb:.2f} s), plain {rate_p:.4g} hist/s, x{rate_b / rate_p:.2f}")
+
+        # ------------------------------------------------ backward cost
+        p3 = mc.Problem.from_string(decks.shield_deck(H, params="24"), xs)
+        e3 = mc.Engine(p3, use_graph=G)
+        n3 = max(2, H // 10)
+        e3.forward(n_histories=n3 // 10)
+        go = np.zeros(e3.n_bins)
+        go[0] = 1.0
+        e3.backward(go, n_histories=n3 // 10)
+        t = time.perf_counter()
+        f3 = e3.forward(n_histories=n3)
+        tf = time.perf_counter() - t
+        t = time.perf_counter()
+        b3 = e3.backward(go, n_histories=n3)
+        tbw = time.perf_counter() - t
+        out["bwd_fwd"] = {"histories": n3, "forward_s": tf, "backward_s": tbw, "ratio": tbw / tf,
+                          "params": int(e3.n_params),
+                          "dT_dtheta_max_rel_sigma": float(np.max(b3.grad_sigma / np.maximum(np.abs(b3.grad), 1e-300)))}
+        log(f"bwd/fwd: {tbw:.3f}/{tf:.3f} = {tbw / tf:.2f}")
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -180,8 +180,18 @@ typedef struct mc_engine_cfg {
     int32_t iters_per_graph; /* event iterations per captured graph (0: 16)     */
     uint64_t bank_capacity;  /* particle slots (0: 1<<20)                       */
     int32_t use_graph;       /* 1: replay hipGraphs, 0: plain launches          */
-    int32_t pad;
+    int32_t profile;         /* 1: plain launches timed per kernel (mc_engine_kernel_stats) */
 } mc_engine_cfg;
+
+/* Per-kernel device time of the event loop, recorded when cfg.profile = 1:
+ * kernels in order reset, refill_count, refill_scatter, xs_lookup, flight,
+ * event_scatter, surface, collision.                                      */
+#define MC_N_KERNELS 8
+typedef struct mc_kernel_stats {
+    uint64_t launches[MC_N_KERNELS];
+    uint64_t items[MC_N_KERNELS];   /* particles (slots for refill) processed  */
+    double seconds[MC_N_KERNELS];
+} mc_kernel_stats;
 
 enum { MC_RUN_BATCHED = 0, MC_RUN_PLAIN = 1 };
 
@@ -222,6 +232,9 @@ mc_status mc_engine_info_get(const mc_engine* e, mc_engine_info* out);
 /* Bin offset of problem p in the engine-wide tally arrays. */
 mc_status mc_engine_bin_offset(const mc_engine* e, int32_t p, int32_t* offset, int32_t* n_bins);
 mc_status mc_engine_param_offset(const mc_engine* e, int32_t p, int32_t* offset, int32_t* n_params);
+/* Accumulated per-kernel statistics since the last reset (reset != 0 clears
+ * them after reading).  No reference counterpart: measurement entry point. */
+mc_status mc_engine_kernel_stats(mc_engine* e, mc_kernel_stats* out, int32_t reset);
 
 mc_status mc_forward(mc_engine* e, const mc_run_cfg* run, mc_tally_out* out);
 

@@ -34,7 +34,7 @@ MC_RUN_BATCHED, MC_RUN_PLAIN = 0, 1
 class EngineCfg(C.Structure):
     _fields_ = [("device", C.c_int32), ("block", C.c_int32), ("grid_per_cu", C.c_int32),
                 ("iters_per_graph", C.c_int32), ("bank_capacity", C.c_uint64),
-                ("use_graph", C.c_int32), ("pad", C.c_int32)]
+                ("use_graph", C.c_int32), ("profile", C.c_int32)]
 
 
 class TraceRecord(C.Structure):
@@ -55,6 +55,14 @@ class TallyOut(C.Structure):
     _fields_ = [("sum", C.POINTER(C.c_double)), ("sq", C.POINTER(C.c_double)),
                 ("mean", C.POINTER(C.c_double)), ("uncer", C.POINTER(C.c_double)),
                 ("flights", C.c_uint64), ("seconds", C.c_double)]
+
+
+KERNEL_NAMES = ("k_reset", "k_refill_count", "k_refill_scatter", "k_xs_lookup", "k_flight",
+                "k_event_scatter", "k_surface", "k_collision")
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8), ("seconds", C.c_double * 8)]
 
 
 class EngineInfo(C.Structure):
@@ -98,6 +106,7 @@ EXPORTS = {
     "mc_engine_info_get": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),
     "mc_engine_bin_offset": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "mc_engine_param_offset": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "mc_engine_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(KernelStats), C.c_int32]),
     "mc_forward": (C.c_int, [C.c_void_p, C.POINTER(RunCfg), C.POINTER(TallyOut)]),
     "mc_backward": (C.c_int, [C.c_void_p, C.POINTER(RunCfg), C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_double), C.POINTER(TallyOut)]),
@@ -215,12 +224,13 @@ class Engine:
     """Device state for one or many problems (mc_engine_create)."""
 
     def __init__(self, problems, device: int = 0, block: int = 0, grid_per_cu: int = 0,
-                 iters_per_graph: int = 0, bank_capacity: int = 0, use_graph: bool = True):
+                 iters_per_graph: int = 0, bank_capacity: int = 0, use_graph: bool = True,
+                 profile: bool = False):
         if isinstance(problems, Problem):
             problems = [problems]
         self.problems = list(problems)
         arr = (C.c_void_p * len(self.problems))(*[p._h.value for p in self.problems])
-        cfg = EngineCfg(device, block, grid_per_cu, iters_per_graph, bank_capacity, int(use_graph), 0)
+        cfg = EngineCfg(device, block, grid_per_cu, iters_per_graph, bank_capacity, int(use_graph), int(profile))
         h = C.c_void_p()
         _check(lib.mc_engine_create(arr, len(self.problems), C.byref(cfg), C.byref(h)))
         self._h = h
@@ -284,6 +294,14 @@ class Engine:
         _check(lib.mc_backward(self._h, C.byref(run), _dptr(go), _dptr(g), _dptr(gs), C.byref(out)))
         return TallyResult(m[:self.n_bins], u[:self.n_bins], s[:self.n_bins], q[:self.n_bins],
                            out.flights, out.seconds, nh, None, g[:self.n_params], gs[:self.n_params])
+
+    def kernel_stats(self, reset: bool = True) -> dict:
+        """Per-kernel device seconds, launches and particles processed
+        (recorded when the engine was created with profile=True)."""
+        k = KernelStats()
+        _check(lib.mc_engine_kernel_stats(self._h, C.byref(k), int(reset)))
+        return {n: dict(launches=k.launches[i], items=k.items[i], seconds=k.seconds[i])
+                for i, n in enumerate(KERNEL_NAMES)}
 
     def xs_lookup(self, energy, mat) -> np.ndarray:
         E = np.ascontiguousarray(energy, dtype=np.float64)

@@ -136,6 +136,14 @@ mc_status mc_engine_param_offset(const mc_engine* e, int32_t p, int32_t* offset,
     });
 }
 
+mc_status mc_engine_kernel_stats(mc_engine* e, mc_kernel_stats* out, int32_t reset)
+{
+    return guarded([&] {
+        if (!e || !out) throw mc::Error(MC_ERR_ARGUMENT, "null argument");
+        e->E->kernel_stats(out, reset != 0);
+    });
+}
+
 mc_status mc_forward(mc_engine* e, const mc_run_cfg* run, mc_tally_out* out)
 {
     return guarded([&] {

@@ -23,7 +23,10 @@ namespace mc {
 constexpr int kBucketBits = 6;                 // buckets per octave = 64
 constexpr int kBucketShift = 52 - kBucketBits;
 
-enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2 };
+// slot status; flight marks the next event, surface/collision resolve it
+enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4 };
+
+constexpr int kTouched = 8;        // per-slot list of tally bins touched by the history
 
 struct DNuc {            // device nuclide (deduplicated across problems)
     double A;
@@ -47,11 +50,14 @@ struct DProb {
 };
 
 struct Counters {
-    uint32_t nA, nS, nC, pad0;
+    uint32_t nA, nS, nC;          // queue lengths (written by one workgroup, no atomics)
+    uint32_t active_cap;          // slots in use this run: min(capacity, histories)
     uint64_t next_hist;           // next history key to start
     uint64_t end_hist;            // one past the last key of this run
+    uint64_t claimed;             // histories started by the last refill
     uint64_t flights;             // Ntrack
     uint32_t err_lost, err_split;
+    uint64_t tA, tS, tC;          // particles processed by flight / surface / collision
 };
 
 // Everything a kernel needs, passed by value.
@@ -99,6 +105,9 @@ struct KArgs {
     int32_t *pcell, *pprob, *psurf;
     uint32_t* nflight;
     uint8_t* status;
+    uint8_t* tcount;              // touched-bin count per slot (kTouched + 1: overflow)
+    uint32_t* tbins;              // [slot][kTouched] touched bins (problem-local)
+    uint32_t* wg_cnt;             // per-workgroup counts for the two-pass compaction
     double* hist;
     double* S;                    // likelihood-ratio scores [slot][param]
     double* G;                    // per-history gradient  [slot][param]

@@ -21,7 +21,9 @@ namespace mc {
 
 // kernels (mc_kernels.hip)
 __global__ void k_reset(Counters* c);
-template <bool kGrad> __global__ void k_refill(KArgs A);
+__global__ void k_refill_count(KArgs A);
+template <bool kGrad> __global__ void k_refill_scatter(KArgs A);
+__global__ void k_event_scatter(KArgs A);
 __global__ void k_xs_lookup(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
 template <bool kGrad> __global__ void k_surface(KArgs A);
@@ -112,7 +114,8 @@ struct Engine::Impl {
         d_run_hb;
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
-        b_status, b_hist, b_S, b_G, q_A, q_S, q_C, d_ctr;
+        b_status, b_tcount, b_tbins, b_hist, b_S, b_G, q_A, q_S, q_C, d_ctr, d_wgcnt;
+    uint32_t h_active_cap = 0;
     uint64_t bank_cap = 0;
     bool bank_grad = false;
     // tallies
@@ -124,6 +127,8 @@ struct Engine::Impl {
     uint64_t generation = 0, graph_gen[2] = {~0ull, ~0ull};
     const void* graph_trace[2] = {nullptr, nullptr};   // trace pointer captured in each graph
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> kev;     // per-kernel timing events (profile mode)
+    mc_kernel_stats kstats{};
 
     ~Impl()
     {
@@ -132,6 +137,7 @@ struct Engine::Impl {
         if (h_ctr) (void)hipHostFree(h_ctr);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (auto e : kev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -139,7 +145,7 @@ struct Engine::Impl {
     void upload_static();
     void ensure_bank(bool grad);
     KArgs args(bool grad, mc_trace_record* dtrace, uint64_t trace_count) const;
-    void launch_iteration(const KArgs& A, bool grad, hipStream_t st);
+    void launch_iteration(const KArgs& A, bool grad, hipStream_t st, hipEvent_t* ev);
     void run_loop(bool grad, const KArgs& A);
     void run(const mc_run_cfg* run, bool grad, const double* gout, mc_tally_out* out, double* grad_mean,
              double* grad_sigma);
@@ -360,6 +366,7 @@ void Engine::Impl::upload_static()
     t_gsum.alloc(sizeof(double) * std::max(1, n_params));
     t_gsq.alloc(sizeof(double) * std::max(1, n_params));
     d_ctr.alloc(sizeof(Counters));
+    d_wgcnt.alloc(sizeof(uint32_t) * 2 * 8192);   // per-workgroup counts (grid <= 8192 blocks)
     HIP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -377,6 +384,8 @@ void Engine::Impl::ensure_bank(bool grad)
         b_surf.alloc(4 * B);
         b_nfl.alloc(4 * B);
         b_status.alloc(B);
+        b_tcount.alloc(B);
+        b_tbins.alloc(4 * (uint64_t)kTouched * B);
         b_hist.alloc(8 * B * (uint64_t)bin_stride);
         q_A.alloc(4 * B);
         q_S.alloc(4 * B);
@@ -442,6 +451,9 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.psurf = b_surf.as<int32_t>();
     A.nflight = b_nfl.as<uint32_t>();
     A.status = b_status.as<uint8_t>();
+    A.tcount = b_tcount.as<uint8_t>();
+    A.tbins = b_tbins.as<uint32_t>();
+    A.wg_cnt = d_wgcnt.as<uint32_t>();
     A.hist = b_hist.as<double>();
     A.S = grad ? b_S.as<double>() : nullptr;
     A.G = grad ? b_G.as<double>() : nullptr;
@@ -459,44 +471,62 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     return A;
 }
 
-void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st)
+// One event iteration.  With ev (2 * MC_N_KERNELS events) each kernel is
+// bracketed by a start/stop event pair for per-kernel timing.
+void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, hipEvent_t* ev)
 {
     const dim3 g = grid_dim(), b((unsigned)cfg.block);
+    auto mark = [&](int k, int edge) {
+        if (ev) HIP_CHECK(hipEventRecord(ev[2 * k + edge], st));
+    };
+    mark(0, 0);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, A.ctr);
-    if (grad) {
-        hipLaunchKernelGGL(k_refill<true>, g, b, 0, st, A);
-        if (has_tables) {
-            hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
-            hipLaunchKernelGGL((k_flight<true, true>), g, b, 0, st, A);
-        } else {
-            hipLaunchKernelGGL((k_flight<false, true>), g, b, 0, st, A);
-        }
-        hipLaunchKernelGGL(k_surface<true>, g, b, 0, st, A);
-        hipLaunchKernelGGL(k_collision<true>, g, b, 0, st, A);
+    mark(0, 1);
+    mark(1, 0);
+    hipLaunchKernelGGL(k_refill_count, g, b, 0, st, A);
+    mark(1, 1);
+    mark(2, 0);
+    if (grad) hipLaunchKernelGGL(k_refill_scatter<true>, g, b, 0, st, A);
+    else hipLaunchKernelGGL(k_refill_scatter<false>, g, b, 0, st, A);
+    mark(2, 1);
+    mark(3, 0);
+    if (has_tables) hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
+    mark(3, 1);
+    mark(4, 0);
+    if (has_tables) {
+        if (grad) hipLaunchKernelGGL((k_flight<true, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_flight<true, false>), g, b, 0, st, A);
     } else {
-        hipLaunchKernelGGL(k_refill<false>, g, b, 0, st, A);
-        if (has_tables) {
-            hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
-            hipLaunchKernelGGL((k_flight<true, false>), g, b, 0, st, A);
-        } else {
-            hipLaunchKernelGGL((k_flight<false, false>), g, b, 0, st, A);
-        }
-        hipLaunchKernelGGL(k_surface<false>, g, b, 0, st, A);
-        hipLaunchKernelGGL(k_collision<false>, g, b, 0, st, A);
+        if (grad) hipLaunchKernelGGL((k_flight<false, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_flight<false, false>), g, b, 0, st, A);
     }
+    mark(4, 1);
+    mark(5, 0);
+    hipLaunchKernelGGL(k_event_scatter, g, b, 0, st, A);
+    mark(5, 1);
+    mark(6, 0);
+    if (grad) hipLaunchKernelGGL(k_surface<true>, g, b, 0, st, A);
+    else hipLaunchKernelGGL(k_surface<false>, g, b, 0, st, A);
+    mark(6, 1);
+    mark(7, 0);
+    if (grad) hipLaunchKernelGGL(k_collision<true>, g, b, 0, st, A);
+    else hipLaunchKernelGGL(k_collision<false>, g, b, 0, st, A);
+    mark(7, 1);
 }
 
 void Engine::Impl::run_loop(bool grad, const KArgs& A)
 {
     const int gi = grad ? 1 : 0;
     const int K = std::max(1, cfg.iters_per_graph);
-    if (cfg.use_graph) {
+    const bool profile = cfg.profile != 0;
+    const bool use_graph = cfg.use_graph && !profile;
+    if (use_graph) {
         if (!graph[gi] || graph_gen[gi] != generation || graph_iters[gi] != K ||
             graph_trace[gi] != (const void*)A.trace) {
             if (graph[gi]) { HIP_CHECK(hipGraphExecDestroy(graph[gi])); graph[gi] = nullptr; }
             hipGraph_t gr;
             HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream);
+            for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream, nullptr);
             HIP_CHECK(hipStreamEndCapture(stream, &gr));
             HIP_CHECK(hipGraphInstantiate(&graph[gi], gr, nullptr, nullptr, 0));
             HIP_CHECK(hipGraphDestroy(gr));
@@ -505,12 +535,32 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             graph_trace[gi] = (const void*)A.trace;
         }
     }
+    if (profile && kev.empty()) {
+        kev.resize(2 * MC_N_KERNELS);
+        for (auto& e : kev) HIP_CHECK(hipEventCreate(&e));
+    }
     const auto t0 = std::chrono::steady_clock::now();
     const double max_s = getenv("MC_MAX_SECONDS") ? atof(getenv("MC_MAX_SECONDS")) : 3600.0;
     for (uint64_t chunk = 0;; ++chunk) {
-        if (cfg.use_graph) HIP_CHECK(hipGraphLaunch(graph[gi], stream));
-        else
-            for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream);
+        if (use_graph) {
+            HIP_CHECK(hipGraphLaunch(graph[gi], stream));
+        } else if (profile) {
+            for (int k = 0; k < K; ++k) {
+                launch_iteration(A, grad, stream, kev.data());
+                HIP_CHECK(hipEventSynchronize(kev.back()));
+                for (int j = 0; j < MC_N_KERNELS; ++j) {
+                    float ms = 0.f;
+                    HIP_CHECK(hipEventElapsedTime(&ms, kev[2 * j], kev[2 * j + 1]));
+                    kstats.seconds[j] += 1e-3 * ms;
+                    kstats.launches[j] += (j == 3 && !has_tables) ? 0 : 1;
+                }
+                kstats.items[0] += 1;
+                kstats.items[1] += h_active_cap;
+                kstats.items[2] += h_active_cap;
+            }
+        } else {
+            for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream, nullptr);
+        }
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
@@ -519,6 +569,14 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
         if (el > max_s)
             throw Error(MC_ERR_CAPACITY, "event loop exceeded " + std::to_string(max_s) +
                                          " s (MC_MAX_SECONDS); live particles: " + std::to_string(h_ctr->nA));
+    }
+    if (profile) {
+        // event counts: flight (and lookup) saw every live particle, then surface / collision
+        kstats.items[3] += has_tables ? h_ctr->tA : 0;
+        kstats.items[4] += h_ctr->tA;
+        kstats.items[5] += h_ctr->tA;
+        kstats.items[6] += h_ctr->tS;
+        kstats.items[7] += h_ctr->tC;
     }
 }
 
@@ -567,6 +625,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.sum, (uint64_t)std::max(1, n_bins), 0.0);
     hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.sq, (uint64_t)std::max(1, n_bins), 0.0);
     hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.hist, bank_cap * (uint64_t)bin_stride, 0.0);
+    hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.tcount, bank_cap, (uint8_t)0);
     if (grad) {
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.gsum, (uint64_t)std::max(1, n_params), 0.0);
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.gsq, (uint64_t)std::max(1, n_params), 0.0);
@@ -580,6 +639,9 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         Counters c{};
         c.next_hist = plain ? hb[p] : 0;
         c.end_hist = plain ? hb[p + 1] : hb[np];
+        // slots worth scanning: no more than the histories of this run
+        c.active_cap = (uint32_t)std::min<uint64_t>(bank_cap, c.end_hist - c.next_hist);
+        h_active_cap = c.active_cap;
         HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.status, bank_cap, (uint8_t)ST_EMPTY);
         run_loop(grad, A);
@@ -653,6 +715,8 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, d->device));
     d->n_cu = prop.multiProcessorCount;
+    if ((int64_t)d->n_cu * cfg.grid_per_cu > 8192 || cfg.grid_per_cu < 1)
+        throw Error(MC_ERR_ARGUMENT, "grid_per_cu gives more than 8192 workgroups");
     HIP_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
     HIP_CHECK(hipHostMalloc((void**)&d->h_ctr, sizeof(Counters), hipHostMallocDefault));
     HIP_CHECK(hipEventCreate(&d->ev0));
@@ -751,8 +815,18 @@ void Engine::xs_bench(uint64_t n, int iters, uint64_t seed, double* sec, double*
     }
 }
 
+void Engine::kernel_stats(mc_kernel_stats* out, bool reset)
+{
+    if (out) *out = d->kstats;
+    if (reset) d->kstats = mc_kernel_stats{};
+}
+
 void Engine::set_cfg(const mc_engine_cfg& c)
 {
+    if ((int64_t)d->n_cu * c.grid_per_cu > 8192 || c.grid_per_cu < 1 || c.block % 64 || c.block < 64 ||
+        c.block > 1024 || c.bank_capacity == 0 || c.bank_capacity > (1ull << 31))
+        throw Error(MC_ERR_ARGUMENT, "invalid launch configuration");
+    d->cfg.profile = c.profile;
     d->cfg.block = c.block;
     d->cfg.grid_per_cu = c.grid_per_cu;
     d->cfg.iters_per_graph = c.iters_per_graph;

@@ -21,6 +21,7 @@ public:
     void xs_lookup(const double* E, const int32_t* mat, double* out, uint64_t n);
     void xs_bench(uint64_t n, int iters, uint64_t seed, double* seconds, double* checksum);
     void set_cfg(const mc_engine_cfg& cfg);
+    void kernel_stats(mc_kernel_stats* out, bool reset);
     mc_engine_cfg cfg() const;
     int n_problems() const;
     uint64_t deck_histories(int p) const;

@@ -1,31 +1,102 @@
-// mc_kernels.hip — event-based transport kernels for gfx950.
+// mc_kernels.hip — event-based transport kernels for gfx950 (wave64).
 //
 // One event iteration over the structure-of-arrays particle bank:
-//   k_reset      zero the queue counters (graph-safe: a kernel, not a memset)
-//   k_refill     compact live slots into queue A (wave ballot + prefix) and
-//                start new histories in free slots (Simulator::start's
-//                sample loop, handler.cpp:19-36)
-//   k_xs_lookup  Sigma_t(E) for every particle in A (Material::SigmaT,
-//                Material.cpp:50-57) — continuous-energy problems only
-//   k_flight     distance to collision vs nearest surface, move, track-length
-//                tallies; appends to the surface or collision queue
-//                (random_walk + move_particle, general.cpp:40-83, 177-211)
-//   k_surface    surface crossing: boundary conditions, cell search, surface
-//                tallies, importance, weight roulette (general.cpp:89-115,
-//                population_control.cpp:9-49)
-//   k_collision  collision tallies, implicit capture, nuclide selection,
-//                elastic scatter, weight roulette (general.cpp:121-163)
-// A history that ends is closed out where it ends: its per-slot tally row is
+//   k_reset           advance the history counter by the last refill's claims
+//   k_refill_count    per-workgroup counts of live and free slots
+//   k_refill_scatter  prefix over the counts, then queue A = live slots plus
+//                     new histories started in free slots, in slot order
+//                     (Simulator::start's sample loop, handler.cpp:19-36)
+//   k_xs_lookup       Sigma_t(E) for every particle in A (Material::SigmaT,
+//                     Material.cpp:50-57) — continuous-energy problems only
+//   k_flight          distance to collision vs nearest surface, move, track-
+//                     length tallies, event marking + per-workgroup event counts
+//                     (random_walk + move_particle, general.cpp:40-83, 177-211)
+//   k_event_scatter   prefix over the event counts -> surface queue S and
+//                     collision queue C, in slot order
+//   k_surface         surface crossing: boundary conditions, cell search, surface
+//                     tallies, importance, weight roulette (general.cpp:89-115,
+//                     population_control.cpp:9-49)
+//   k_collision       collision tallies, implicit capture, nuclide selection,
+//                     elastic scatter, weight roulette (general.cpp:121-163)
+// Queues are built without atomics: wave ballots rank lanes inside a block,
+// blocks rank among themselves from per-workgroup counts, so every queue keeps
+// slot order and consecutive lanes touch consecutive bank entries.
+// A history that ends is closed out where it ends: the bins it touched are
 // added to the FP64 moment arrays (Estimator::end_history, Estimator.cpp:339-346).
 #include "mc_device.hpp"
 
 namespace mc {
 
-#define MC_WAVE_LOOP(count)                                                              \
-    const uint32_t _lane = lane_id();                                                    \
-    const uint32_t _wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;                 \
-    const uint32_t _nwaves = (gridDim.x * blockDim.x) >> 6;                              \
-    for (uint32_t _base = _wave * 64u; _base < (count); _base += _nwaves * 64u)
+// ------------------------------------------------------------ block helpers
+
+// Rank of a set flag among the block's set flags (in thread order) and the
+// block total.  lds: at least 16 words.
+__device__ __forceinline__ uint32_t block_flag_rank(bool f, uint32_t& total, uint32_t* lds)
+{
+    const uint64_t m = __ballot(f);
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (uint32_t k = 0; k < nw; ++k) {
+        const uint32_t c = lds[k];
+        before += k < wave ? c : 0u;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Sum over the block of two counters.  lds: at least 32 words.
+__device__ __forceinline__ void block_sum2(uint32_t& a, uint32_t& b, uint32_t* lds)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) { lds[2 * wave] = a; lds[2 * wave + 1] = b; }
+    __syncthreads();
+    a = 0;
+    b = 0;
+    for (uint32_t k = 0; k < nw; ++k) { a += lds[2 * k]; b += lds[2 * k + 1]; }
+    __syncthreads();
+}
+
+// Block-contiguous partition of n items over the grid, in whole blocks.
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t& lo, uint32_t& hi)
+{
+    const uint32_t nb = gridDim.x, bd = blockDim.x;
+    uint32_t per = (n + nb - 1) / nb;
+    per = (per + bd - 1) / bd * bd;
+    lo = blockIdx.x * per;
+    hi = lo + per < n ? lo + per : n;
+    if (lo > n) lo = n;
+}
+
+// Prefix over per-workgroup count pairs: sums of workgroups before this one and
+// grand totals.
+__device__ __forceinline__ void wg_prefix(const uint32_t* cnt, uint32_t& a_before, uint32_t& b_before,
+                                          uint32_t& a_tot, uint32_t& b_tot, uint32_t* lds)
+{
+    uint32_t ab = 0, bb = 0, at = 0, bt = 0;
+    for (uint32_t v = threadIdx.x; v < gridDim.x; v += blockDim.x) {
+        const uint32_t a = cnt[2 * v], b = cnt[2 * v + 1];
+        if (v < blockIdx.x) { ab += a; bb += b; }
+        at += a;
+        bt += b;
+    }
+    block_sum2(ab, bb, lds);
+    block_sum2(at, bt, lds);
+    a_before = ab;
+    b_before = bb;
+    a_tot = at;
+    b_tot = bt;
+}
+
+#define MC_GRID_LOOP(count)                                                                \
+    for (uint32_t _i = blockIdx.x * blockDim.x + threadIdx.x; _i < (count); _i += gridDim.x * blockDim.x)
 
 // ---------------------------------------------------------------- helpers
 
@@ -63,10 +134,28 @@ __device__ __forceinline__ int find_cell(const KArgs& A, int s, const DProb& PR,
     return -1;
 }
 
+// Adds v to the history's tally bin b (problem-local) and records the bin the
+// first time it becomes non-zero.
+__device__ __forceinline__ void hist_add(const KArgs& A, uint32_t s, int b, double v)
+{
+    double* h = A.hist + (size_t)s * (size_t)A.bin_stride + b;
+    const double old = *h;
+    *h = old + v;
+    if (old == 0.0 && v != 0.0) {
+        const uint8_t n = A.tcount[s];
+        if (n < kTouched) {
+            A.tbins[(size_t)s * kTouched + n] = (uint32_t)b;
+            A.tcount[s] = n + 1;
+        } else if (n == kTouched) {
+            A.tcount[s] = kTouched + 1;      // overflow: close out by full scan
+        }
+    }
+}
+
 // Estimator::score (Estimator.cpp:298-336) for one attachment.
 template <bool kGrad>
-__device__ void score_attachment(const KArgs& A, int att, double l, int cellid, double E, double w,
-                                 const DProb& PR, double* hrow, const double* Srow, double* Grow)
+__device__ void score_attachment(const KArgs& A, uint32_t s, int att, double l, int cellid, double E, double w,
+                                 const DProb& PR, const double* Srow, double* Grow)
 {
     const int e = A.att_est[att];
     const mc_estimator_desc& Es = A.est[e];
@@ -100,7 +189,7 @@ __device__ void score_attachment(const KArgs& A, int att, double l, int cellid, 
         else if (sc == MC_SCORE_TOTAL) { sx = X.t; v = X.t * kv; }
         else v = kv;
         const int bin = Es.bin_offset + sidx * natt * NE + fi * NE + ebin;
-        hrow[bin - PR.bin_begin] += v;
+        hist_add(A, s, bin - PR.bin_begin, v);
         if (kGrad) {
             const double g = A.gout[bin];
             if (g != 0.0 && v != 0.0) {
@@ -124,22 +213,47 @@ __device__ void score_attachment(const KArgs& A, int att, double l, int cellid, 
     }
 }
 
+__device__ __forceinline__ void close_bin(const KArgs& A, double* row, const DProb& PR, int b)
+{
+    const double h = row[b];
+    if (h != 0.0) {
+        atomicAdd(&A.sum[PR.bin_begin + b], h);
+        atomicAdd(&A.sq[PR.bin_begin + b], h * h);
+        row[b] = 0.0;
+    }
+}
+
 // End of a history: close out its tally row (Estimator::end_history).
 template <bool kGrad>
 __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
                                double E, uint32_t nfl)
 {
     double* row = A.hist + (size_t)s * (size_t)A.bin_stride;
+    const uint32_t nt = A.tcount[s];
+    const uint32_t* tb = A.tbins + (size_t)s * kTouched;
     double tot = 0.0;
-    for (int b = 0; b < PR.bin_count; ++b) {
-        const double h = row[b];
-        tot += h;
-        if (h != 0.0) {
-            atomicAdd(&A.sum[PR.bin_begin + b], h);
-            atomicAdd(&A.sq[PR.bin_begin + b], h * h);
-            row[b] = 0.0;
+    if (A.trace) {
+        // sum of the history's bins in ascending bin order (the oracle's order)
+        if (nt <= kTouched) {
+            int list[kTouched];
+            for (uint32_t j = 0; j < nt; ++j) list[j] = tb[j];
+            for (uint32_t j = 1; j < nt; ++j)
+                for (uint32_t k = j; k > 0 && list[k - 1] > list[k]; --k) {
+                    const int t = list[k]; list[k] = list[k - 1]; list[k - 1] = t;
+                }
+            int prev = -1;
+            for (uint32_t j = 0; j < nt; ++j)
+                if (list[j] != prev) { tot += row[list[j]]; prev = list[j]; }
+        } else {
+            for (int b = 0; b < PR.bin_count; ++b) tot += row[b];
         }
     }
+    if (nt <= kTouched) {
+        for (uint32_t j = 0; j < nt; ++j) close_bin(A, row, PR, tb[j]);
+    } else {
+        for (int b = 0; b < PR.bin_count; ++b) close_bin(A, row, PR, b);
+    }
+    A.tcount[s] = 0;
     atomicAdd((unsigned long long*)&A.ctr->flights, (unsigned long long)nfl);
     if (A.trace) {
         const uint64_t local = A.hid[s] - A.run_hb[p];
@@ -169,36 +283,71 @@ __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& P
 
 __global__ void k_reset(Counters* c)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) { c->nA = 0; c->nS = 0; c->nC = 0; }
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->tA += c->nA;            // event counts of the previous iteration
+        c->tS += c->nS;
+        c->tC += c->nC;
+        c->next_hist += c->claimed;
+        c->claimed = 0;
+        c->nS = 0;
+        c->nC = 0;
+    }
 }
 
-template <bool kGrad>
-__global__ void __launch_bounds__(1024) k_refill(KArgs A)
+// Pass 1 of the refill: live and free slots per workgroup.
+__global__ void __launch_bounds__(1024) k_refill_count(KArgs A)
 {
-    const uint32_t cap = (uint32_t)A.cap;
-    const uint64_t end_h = A.ctr->end_hist;
-    MC_WAVE_LOOP(cap)
-    {
-        const uint32_t i = _base + _lane;
-        const bool valid = i < cap;
+    __shared__ uint32_t lds[32];
+    uint32_t lo, hi;
+    block_range(A.ctr->active_cap, lo, hi);
+    uint32_t live = 0, freec = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint8_t st = A.status[i];
+        live += st == ST_ALIVE;
+        freec += st == ST_DEAD || st == ST_EMPTY;
+    }
+    block_sum2(live, freec, lds);
+    if (threadIdx.x == 0) {
+        A.wg_cnt[2 * blockIdx.x] = live;
+        A.wg_cnt[2 * blockIdx.x + 1] = freec;
+    }
+}
+
+// Pass 2 of the refill: slot-ordered queue A and new histories, no atomics.
+template <bool kGrad>
+__global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
+{
+    __shared__ uint32_t lds[32];
+    const uint32_t cap = A.ctr->active_cap;
+    const uint64_t next = A.ctr->next_hist, end = A.ctr->end_hist;
+    const uint64_t remaining = end > next ? end - next : 0;
+    uint32_t live_before, free_before, live_tot, free_tot;
+    wg_prefix(A.wg_cnt, live_before, free_before, live_tot, free_tot, lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint64_t claim = (uint64_t)free_tot < remaining ? (uint64_t)free_tot : remaining;
+        A.ctr->claimed = claim;
+        A.ctr->nA = live_tot + (uint32_t)claim;
+    }
+    uint32_t lo, hi;
+    block_range(cap, lo, hi);
+    // slots of this block that are free and get a history: free rank < remaining
+    uint64_t free_run = free_before;               // free slots before the current tile
+    uint64_t act_run = live_before + (free_before < remaining ? free_before : remaining);
+    for (uint32_t base = lo; base < hi; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < hi;
         const uint8_t st = valid ? A.status[i] : ST_ALIVE;
         const bool keep = valid && st == ST_ALIVE;
-        const bool need = valid && st != ST_ALIVE;
-        // claim history keys for free slots: one atomic per wave
-        const uint64_t mask = __ballot(need);
-        bool got = false;
-        uint64_t hid = 0;
-        if (mask != 0 && __atomic_load_n(&A.ctr->next_hist, __ATOMIC_RELAXED) < end_h) {
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            unsigned long long base = 0;
-            if ((int)_lane == leader)
-                base = atomicAdd((unsigned long long*)&A.ctr->next_hist, (unsigned long long)__popcll(mask));
-            base = __shfl(base, leader);
-            hid = base + (uint64_t)__popcll(mask & ((1ull << _lane) - 1ull));
-            got = need && hid < end_h;
-        }
+        const bool need = valid && (st == ST_DEAD || st == ST_EMPTY);
+        uint32_t nfree;
+        const uint32_t frank = block_flag_rank(need, nfree, lds);
+        const uint64_t fidx = free_run + frank;
+        const bool got = need && fidx < remaining;
+        uint32_t nact;
+        const uint32_t arank = block_flag_rank(keep || got, nact, lds);
         if (got) {
             // SourceBank::get_source + SourcePoint::get_source (Source.cpp:20-45)
+            const uint64_t hid = next + fidx;
             const int p = find_problem(A, hid);
             const DProb PR = A.prob[p];
             uint64_t rng = history_seed(PR.seed, hid, A.jump);
@@ -232,6 +381,7 @@ __global__ void __launch_bounds__(1024) k_refill(KArgs A)
             A.pcell[i] = S.cell;
             A.pprob[i] = p;
             A.nflight[i] = 0;
+            A.tcount[i] = 0;
             A.status[i] = ST_ALIVE;
             if (kGrad) {
                 double* Srow = A.S + (size_t)i * (size_t)A.par_stride;
@@ -240,107 +390,129 @@ __global__ void __launch_bounds__(1024) k_refill(KArgs A)
         } else if (need && st != ST_EMPTY) {
             A.status[i] = ST_EMPTY;
         }
-        const bool active = keep || got;
-        const uint32_t pos = wave_append(active, &A.ctr->nA);
-        if (active) A.qA[pos] = i;
+        if (keep || got) A.qA[act_run + arank] = i;
+        free_run += nfree;
+        act_run += nact;
     }
 }
 
 __global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A)
 {
     const uint32_t n = A.ctr->nA;
-    MC_WAVE_LOOP(n)
+    MC_GRID_LOOP(n)
     {
-        const uint32_t i = _base + _lane;
-        if (i < n) {
-            const uint32_t s = A.qA[i];
-            const int m = A.cell[A.pcell[s]].material;
-            A.sigt[s] = m >= 0 ? material_sigt(A, m, A.E[s]) : 0.0;
-        }
+        const uint32_t s = A.qA[_i];
+        const int m = A.cell[A.pcell[s]].material;
+        A.sigt[s] = m >= 0 ? material_sigt(A, m, A.E[s]) : 0.0;
     }
 }
 
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
+    __shared__ uint32_t lds[32];
     const uint32_t n = A.ctr->nA;
-    MC_WAVE_LOOP(n)
-    {
-        const uint32_t i = _base + _lane;
-        const bool valid = i < n;
-        bool hit = false;
-        uint32_t s = 0;
-        if (valid) {
-            s = A.qA[i];
-            const int c = A.pcell[s];
-            const mc_cell_desc C = A.cell[c];
-            Vec3 p = {A.x[s], A.y[s], A.z[s]};
-            const Vec3 u = {A.u[s], A.v[s], A.w[s]};
-            // surface_intersect, general.cpp:54-67
-            double dist = kMaxFloat;
-            int S = -1;
-            for (int k = 0; k < C.surf_count; ++k) {
-                const int sf = A.cell_surf[C.surf_begin + k];
-                const mc_surface_desc& Sd = A.surf[sf];
-                const double d = surface_distance(Sd.type, Sd.c, p, u);
-                if (d < dist) { dist = d; S = sf; }
-            }
-            // collision_distance, general.cpp:40-48
-            uint64_t rng = A.rng[s];
-            const double E = A.E[s];
-            double dcol, sigt = 0.0;
-            if (C.material >= 0) {
-                sigt = kTables ? A.sigt[s] : material_sigt(A, C.material, E);
-                dcol = -mc_log(urand(rng)) / sigt;
-            } else {
-                dcol = kMaxFloatLess;
-            }
-            hit = dcol > dist;
-            const double l = hit ? dist : dcol;
-            const int p_id = A.pprob[s];
-            const DProb PR = A.prob[p_id];
-            if (kGrad && C.material >= 0) {
-                const mc_material_desc M = A.mat[C.material];
-                double* Srow = A.S + (size_t)s * (size_t)A.par_stride;
-                XsPoint P;
-                for (int q = 0; q < M.param_count; ++q) {
-                    const DSig d = param_dsig(A, M, q, E, P);
-                    const double dt = d.ds + d.dc;
-                    Srow[d.k - PR.param_begin] += -l * dt + (hit ? 0.0 : dt / sigt);
-                }
-            }
-            // move_particle, general.cpp:73-83
-            p.x += u.x * l;
-            p.y += u.y * l;
-            p.z += u.z * l;
-            const uint32_t nfl = A.nflight[s] + 1;
-            const double w = A.wgt[s];
-            if (C.tl_count > 0) {
-                double* hrow = A.hist + (size_t)s * (size_t)A.bin_stride;
-                const double* Srow = kGrad ? A.S + (size_t)s * (size_t)A.par_stride : nullptr;
-                double* Grow = kGrad ? A.G + (size_t)s * (size_t)A.par_stride : nullptr;
-                for (int a = 0; a < C.tl_count; ++a)
-                    score_attachment<kGrad>(A, C.tl_begin + a, l, c, E, w, PR, hrow, Srow, Grow);
-            }
-            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-            A.rng[s] = rng;
-            A.nflight[s] = nfl;
-            A.psurf[s] = S;
-            if (hit && S < 0) {            // no surface ahead: lost particle
-                atomicAdd(&A.ctr->err_lost, 1u);
-                A.wgt[s] = 0.0;
-                finish_history<kGrad>(A, s, p_id, PR, rng, p, E, nfl);
-                A.status[s] = ST_DEAD;
-                hit = false;
-                s = 0xffffffffu;
+    uint32_t lo, hi;
+    block_range(n, lo, hi);
+    uint32_t nsurf = 0, ncoll = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint32_t s = A.qA[i];
+        const int c = A.pcell[s];
+        const mc_cell_desc C = A.cell[c];
+        Vec3 p = {A.x[s], A.y[s], A.z[s]};
+        const Vec3 u = {A.u[s], A.v[s], A.w[s]};
+        // surface_intersect, general.cpp:54-67
+        double dist = kMaxFloat;
+        int S = -1;
+        for (int k = 0; k < C.surf_count; ++k) {
+            const int sf = A.cell_surf[C.surf_begin + k];
+            const mc_surface_desc& Sd = A.surf[sf];
+            const double d = surface_distance(Sd.type, Sd.c, p, u);
+            if (d < dist) { dist = d; S = sf; }
+        }
+        // collision_distance, general.cpp:40-48
+        uint64_t rng = A.rng[s];
+        const double E = A.E[s];
+        double dcol, sigt = 0.0;
+        if (C.material >= 0) {
+            sigt = kTables ? A.sigt[s] : material_sigt(A, C.material, E);
+            dcol = -mc_log(urand(rng)) / sigt;
+        } else {
+            dcol = kMaxFloatLess;
+        }
+        const bool hit = dcol > dist;
+        const double l = hit ? dist : dcol;
+        const int p_id = A.pprob[s];
+        const DProb PR = A.prob[p_id];
+        if (kGrad && C.material >= 0) {
+            const mc_material_desc M = A.mat[C.material];
+            double* Srow = A.S + (size_t)s * (size_t)A.par_stride;
+            XsPoint P;
+            for (int q = 0; q < M.param_count; ++q) {
+                const DSig d = param_dsig(A, M, q, E, P);
+                const double dt = d.ds + d.dc;
+                Srow[d.k - PR.param_begin] += -l * dt + (hit ? 0.0 : dt / sigt);
             }
         }
-        const bool to_surf = valid && hit;
-        const bool to_coll = valid && !hit && s != 0xffffffffu;
-        const uint32_t ps = wave_append(to_surf, &A.ctr->nS);
-        if (to_surf) A.qS[ps] = s;
-        const uint32_t pc = wave_append(to_coll, &A.ctr->nC);
-        if (to_coll) A.qC[pc] = s;
+        // move_particle, general.cpp:73-83
+        p.x += u.x * l;
+        p.y += u.y * l;
+        p.z += u.z * l;
+        const uint32_t nfl = A.nflight[s] + 1;
+        const double w = A.wgt[s];
+        if (C.tl_count > 0) {
+            const double* Srow = kGrad ? A.S + (size_t)s * (size_t)A.par_stride : nullptr;
+            double* Grow = kGrad ? A.G + (size_t)s * (size_t)A.par_stride : nullptr;
+            for (int a = 0; a < C.tl_count; ++a)
+                score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, Srow, Grow);
+        }
+        A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+        A.rng[s] = rng;
+        A.nflight[s] = nfl;
+        A.psurf[s] = S;
+        uint8_t st = hit ? ST_SURF : ST_COLL;
+        if (hit && S < 0) {            // no surface ahead: lost particle
+            atomicAdd(&A.ctr->err_lost, 1u);
+            A.wgt[s] = 0.0;
+            finish_history<kGrad>(A, s, p_id, PR, rng, p, E, nfl);
+            st = ST_DEAD;
+        }
+        A.status[s] = st;
+        nsurf += st == ST_SURF;
+        ncoll += st == ST_COLL;
+    }
+    block_sum2(nsurf, ncoll, lds);
+    if (threadIdx.x == 0) {
+        A.wg_cnt[2 * blockIdx.x] = nsurf;
+        A.wg_cnt[2 * blockIdx.x + 1] = ncoll;
+    }
+}
+
+// Surface and collision queues from the flight's marks, in A's order.
+__global__ void __launch_bounds__(1024) k_event_scatter(KArgs A)
+{
+    __shared__ uint32_t lds[32];
+    const uint32_t n = A.ctr->nA;
+    uint32_t s_before, c_before, s_tot, c_tot;
+    wg_prefix(A.wg_cnt, s_before, c_before, s_tot, c_tot, lds);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.ctr->nS = s_tot;
+        A.ctr->nC = c_tot;
+    }
+    uint32_t lo, hi;
+    block_range(n, lo, hi);
+    uint32_t srun = s_before, crun = c_before;
+    for (uint32_t base = lo; base < hi; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t s = i < hi ? A.qA[i] : 0u;
+        const uint8_t st = i < hi ? A.status[s] : ST_DEAD;
+        uint32_t ns, nc;
+        const uint32_t rs = block_flag_rank(st == ST_SURF, ns, lds);
+        const uint32_t rc = block_flag_rank(st == ST_COLL, nc, lds);
+        if (st == ST_SURF) A.qS[srun + rs] = s;
+        if (st == ST_COLL) A.qC[crun + rc] = s;
+        srun += ns;
+        crun += nc;
     }
 }
 
@@ -357,11 +529,9 @@ template <bool kGrad>
 __global__ void __launch_bounds__(1024) k_surface(KArgs A)
 {
     const uint32_t n = A.ctr->nS;
-    MC_WAVE_LOOP(n)
+    MC_GRID_LOOP(n)
     {
-        const uint32_t i = _base + _lane;
-        if (i >= n) continue;
-        const uint32_t s = A.qS[i];
+        const uint32_t s = A.qS[_i];
         const int sf = A.psurf[s];
         const mc_surface_desc& Sd = A.surf[sf];
         const int p_id = A.pprob[s];
@@ -396,11 +566,10 @@ __global__ void __launch_bounds__(1024) k_surface(KArgs A)
             p.z += u.z * kEpsFloat;
         }
         if (Sd.est_count > 0) {
-            double* hrow = A.hist + (size_t)s * (size_t)A.bin_stride;
             const double* Srow = kGrad ? A.S + (size_t)s * (size_t)A.par_stride : nullptr;
             double* Grow = kGrad ? A.G + (size_t)s * (size_t)A.par_stride : nullptr;
             for (int a = 0; a < Sd.est_count; ++a)
-                score_attachment<kGrad>(A, Sd.est_begin + a, 0.0, cell, E, w, PR, hrow, Srow, Grow);
+                score_attachment<kGrad>(A, s, Sd.est_begin + a, 0.0, cell, E, w, PR, Srow, Grow);
         }
         // cell_importance, population_control.cpp:21-49
         const double Iold = A.cell[cell_old].importance;
@@ -435,11 +604,9 @@ template <bool kGrad>
 __global__ void __launch_bounds__(1024) k_collision(KArgs A)
 {
     const uint32_t n = A.ctr->nC;
-    MC_WAVE_LOOP(n)
+    MC_GRID_LOOP(n)
     {
-        const uint32_t i = _base + _lane;
-        if (i >= n) continue;
-        const uint32_t s = A.qC[i];
+        const uint32_t s = A.qC[_i];
         const int c = A.pcell[s];
         const mc_cell_desc C = A.cell[c];
         const int p_id = A.pprob[s];
@@ -454,11 +621,10 @@ __global__ void __launch_bounds__(1024) k_collision(KArgs A)
             w = 0.0;
         } else {
             if (C.col_count > 0) {
-                double* hrow = A.hist + (size_t)s * (size_t)A.bin_stride;
                 const double* Srow = kGrad ? A.S + (size_t)s * (size_t)A.par_stride : nullptr;
                 double* Grow = kGrad ? A.G + (size_t)s * (size_t)A.par_stride : nullptr;
                 for (int a = 0; a < C.col_count; ++a)
-                    score_attachment<kGrad>(A, C.col_begin + a, 0.0, c, E, w, PR, hrow, Srow, Grow);
+                    score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, Srow, Grow);
             }
             const int m = C.material;
             const Xs3 X = material_xs3(A, m, E);
@@ -577,8 +743,8 @@ __global__ void k_selftest_math(const double* in, double* out, uint64_t n)
 }
 
 // explicit instantiations used by the host
-template __global__ void k_refill<false>(KArgs);
-template __global__ void k_refill<true>(KArgs);
+template __global__ void k_refill_scatter<false>(KArgs);
+template __global__ void k_refill_scatter<true>(KArgs);
 template __global__ void k_flight<false, false>(KArgs);
 template __global__ void k_flight<true, false>(KArgs);
 template __global__ void k_flight<false, true>(KArgs);

@@ -96,12 +96,12 @@ def batched_slabs(n: int = 4096, seed: int = 2024):
 
 
 # --------------------------------------------------------- continuous energy
-# (name, A, potential scatter [b], 1/v scatter at 0.0253 eV [b], 1/v capture at 0.0253 eV [b])
+# (name, A, constant scatter [b], 1/v scatter at 0.0253 eV [b], 1/v capture at 0.0253 eV [b])
 SHIELD_NUCLIDES = [
-    ("H1x", 1.0, 20.0, 10.0, 0.332),
-    ("C12x", 12.0, 4.7, 0.3, 0.0035),
-    ("O16x", 16.0, 3.8, 0.2, 0.00019),
-    ("Fe56x", 56.0, 11.0, 1.0, 2.56),
+    ("H1x", 1.0, 3.0, 17.0, 0.332),
+    ("C12x", 12.0, 1.7, 3.0, 0.0035),
+    ("O16x", 16.0, 1.2, 2.6, 0.00019),
+    ("Fe56x", 56.0, 3.0, 8.0, 2.56),
 ]
 # atom densities (atoms / barn cm); every layer holds all four nuclides
 SHIELD_MATERIALS = {
