@@ -113,12 +113,15 @@ struct KArgs {
     double* G;                    // per-history gradient  [slot][param]
     uint32_t *qA, *qS, *qC;
     Counters* ctr;
-    // tallies
-    double* sum;
-    double* sq;
+    // tallies: per-workgroup partial moments [workgroup][bin], reduced at the
+    // end of a run; a row stride of 0 makes every workgroup share one row
+    double* psum;
+    double* psq;
+    uint64_t bin_row;             // row stride of psum/psq (engine bins, or 0)
     const double* gout;           // gradient weights per bin
-    double* gsum;
-    double* gsq;
+    double* pgsum;
+    double* pgsq;
+    uint64_t par_row;             // row stride of pgsum/pgsq (engine params, or 0)
     mc_trace_record* trace;       // device buffer [prob][trace_count]
     uint64_t trace_count;
 };

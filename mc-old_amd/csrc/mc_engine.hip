@@ -32,6 +32,7 @@ __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, 
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
                             const int32_t* mats, int32_t nmat, uint64_t seed);
 __global__ void k_fill_f64(double* p, uint64_t n, double v);
+__global__ void k_reduce_rows(const double* part, uint64_t rows, uint64_t n, double* out);
 __global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v);
 __global__ void k_selftest_math(const double* in, double* out, uint64_t n);
 
@@ -120,6 +121,8 @@ struct Engine::Impl {
     bool bank_grad = false;
     // tallies
     DevBuf t_sum, t_sq, t_gout, t_gsum, t_gsq, t_trace;
+    DevBuf t_psum, t_psq, t_pgsum, t_pgsq;   // per-workgroup partial moments
+    uint64_t part_rows = 0;
     Counters* h_ctr = nullptr;   // pinned
     // graphs
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -144,6 +147,7 @@ struct Engine::Impl {
     void build(const std::vector<const Problem*>& ps);
     void upload_static();
     void ensure_bank(bool grad);
+    void ensure_partials();
     KArgs args(bool grad, mc_trace_record* dtrace, uint64_t trace_count) const;
     void launch_iteration(const KArgs& A, bool grad, hipStream_t st, hipEvent_t* ev);
     void run_loop(bool grad, const KArgs& A);
@@ -401,6 +405,22 @@ void Engine::Impl::ensure_bank(bool grad)
     ++generation;
 }
 
+// One private row of tally moments per workgroup when that fits in 2^26
+// entries per array, else a single shared row (stride 0).
+void Engine::Impl::ensure_partials()
+{
+    const uint64_t wgs = (uint64_t)grid_dim().x;
+    const uint64_t width = (uint64_t)std::max(1, std::max(n_bins, n_params));
+    const uint64_t rows = wgs * width <= (1ull << 26) ? wgs : 1;
+    if (rows == part_rows) return;
+    t_psum.alloc(8 * rows * (uint64_t)std::max(1, n_bins));
+    t_psq.alloc(8 * rows * (uint64_t)std::max(1, n_bins));
+    t_pgsum.alloc(8 * rows * (uint64_t)std::max(1, n_params));
+    t_pgsq.alloc(8 * rows * (uint64_t)std::max(1, n_params));
+    part_rows = rows;
+    ++generation;
+}
+
 KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_count) const
 {
     KArgs A{};
@@ -461,11 +481,13 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.qS = q_S.as<uint32_t>();
     A.qC = q_C.as<uint32_t>();
     A.ctr = d_ctr.as<Counters>();
-    A.sum = t_sum.as<double>();
-    A.sq = t_sq.as<double>();
+    A.psum = t_psum.as<double>();
+    A.psq = t_psq.as<double>();
+    A.bin_row = part_rows > 1 ? (uint64_t)n_bins : 0;
     A.gout = t_gout.as<double>();
-    A.gsum = t_gsum.as<double>();
-    A.gsq = t_gsq.as<double>();
+    A.pgsum = t_pgsum.as<double>();
+    A.pgsq = t_pgsq.as<double>();
+    A.par_row = part_rows > 1 ? (uint64_t)n_params : 0;
     A.trace = dtrace;
     A.trace_count = trace_count;
     return A;
@@ -620,15 +642,17 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         HIP_CHECK(hipMemsetAsync(t_trace.p, 0, bytes, stream));
         dtrace = t_trace.as<mc_trace_record>();
     }
+    ensure_partials();
     const KArgs A = args(grad, dtrace, tcount);
     const dim3 fg(1024), fb(256);
-    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.sum, (uint64_t)std::max(1, n_bins), 0.0);
-    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.sq, (uint64_t)std::max(1, n_bins), 0.0);
+    const uint64_t nbp = part_rows * (uint64_t)std::max(1, n_bins), npp = part_rows * (uint64_t)std::max(1, n_params);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.psum, nbp, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.psq, nbp, 0.0);
     hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.hist, bank_cap * (uint64_t)bin_stride, 0.0);
     hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.tcount, bank_cap, (uint8_t)0);
     if (grad) {
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.gsum, (uint64_t)std::max(1, n_params), 0.0);
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.gsq, (uint64_t)std::max(1, n_params), 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsum, npp, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsq, npp, 0.0);
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.G, bank_cap * (uint64_t)par_stride, 0.0);
     }
     HIP_CHECK(hipEventRecord(ev0, stream));
@@ -645,7 +669,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.status, bank_cap, (uint8_t)ST_EMPTY);
         run_loop(grad, A);
-        flights += h_ctr->flights;
+        flights += h_ctr->tA;      // every entry of queue A is one flight
         err_lost += h_ctr->err_lost;
         err_split += h_ctr->err_split;
     }
@@ -656,9 +680,20 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     if (err_lost) throw Error(MC_ERR_LOST_PARTICLE, std::to_string(err_lost) + " particle(s) lost");
     if (err_split)
         throw Error(MC_ERR_UNSUPPORTED, "importance splitting (ratio > 1) is not supported by the GPU engine");
+    // reduce the per-workgroup partial moments (workgroups in order)
+    const dim3 rg(256), rb(256);
+    hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.psum, part_rows, (uint64_t)n_bins, t_sum.as<double>());
+    hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.psq, part_rows, (uint64_t)n_bins, t_sq.as<double>());
+    if (grad) {
+        hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.pgsum, part_rows, (uint64_t)n_params,
+                           t_gsum.as<double>());
+        hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.pgsq, part_rows, (uint64_t)n_params,
+                           t_gsq.as<double>());
+    }
+    HIP_CHECK(hipStreamSynchronize(stream));
     std::vector<double> sum(n_bins), sq(n_bins);
-    HIP_CHECK(hipMemcpy(sum.data(), A.sum, sizeof(double) * n_bins, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(sq.data(), A.sq, sizeof(double) * n_bins, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(sum.data(), t_sum.p, sizeof(double) * n_bins, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(sq.data(), t_sq.p, sizeof(double) * n_bins, hipMemcpyDeviceToHost));
     if (out) {
         out->flights = flights;
         out->seconds = ms * 1e-3;
@@ -677,8 +712,8 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     }
     if (grad) {
         std::vector<double> gs(n_params), gq(n_params);
-        HIP_CHECK(hipMemcpy(gs.data(), A.gsum, sizeof(double) * n_params, hipMemcpyDeviceToHost));
-        HIP_CHECK(hipMemcpy(gq.data(), A.gsq, sizeof(double) * n_params, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(gs.data(), t_gsum.p, sizeof(double) * n_params, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(gq.data(), t_gsq.p, sizeof(double) * n_params, hipMemcpyDeviceToHost));
         for (int p = 0; p < np; ++p) {
             const double N = (double)nh[p];
             for (int k = prob[p].param_begin; k < prob[p].param_begin + prob[p].param_count; ++k) {

@@ -217,8 +217,10 @@ __device__ __forceinline__ void close_bin(const KArgs& A, double* row, const DPr
 {
     const double h = row[b];
     if (h != 0.0) {
-        atomicAdd(&A.sum[PR.bin_begin + b], h);
-        atomicAdd(&A.sq[PR.bin_begin + b], h * h);
+        // this workgroup's private row: no contention with other workgroups
+        const size_t k = (size_t)blockIdx.x * A.bin_row + (size_t)(PR.bin_begin + b);
+        atomicAdd(&A.psum[k], h);
+        atomicAdd(&A.psq[k], h * h);
         row[b] = 0.0;
     }
 }
@@ -254,7 +256,6 @@ __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& P
         for (int b = 0; b < PR.bin_count; ++b) close_bin(A, row, PR, b);
     }
     A.tcount[s] = 0;
-    atomicAdd((unsigned long long*)&A.ctr->flights, (unsigned long long)nfl);
     if (A.trace) {
         const uint64_t local = A.hid[s] - A.run_hb[p];
         if (local < A.trace_count) {
@@ -271,8 +272,9 @@ __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& P
         for (int k = 0; k < PR.param_count; ++k) {
             const double g = G[k];
             if (g != 0.0) {
-                atomicAdd(&A.gsum[PR.param_begin + k], g);
-                atomicAdd(&A.gsq[PR.param_begin + k], g * g);
+                const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
+                atomicAdd(&A.pgsum[j], g);
+                atomicAdd(&A.pgsq[j], g * g);
                 G[k] = 0.0;
             }
         }
@@ -712,6 +714,17 @@ __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, d
         E[i] = exp(lnmin + a * lnspan);
         int k = (int)(b * (double)nmat);
         mat[i] = mats[k < nmat ? k : nmat - 1];
+    }
+}
+
+// out[b] = sum over rows w of part[w * n + b] (rows in order)
+__global__ void k_reduce_rows(const double* part, uint64_t rows, uint64_t n, double* out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += stride) {
+        double s = 0.0;
+        for (uint64_t w = 0; w < rows; ++w) s += part[w * n + b];
+        out[b] = s;
     }
 }
 
