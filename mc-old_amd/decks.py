@@ -29,8 +29,10 @@ def _f(x: float) -> str:
 
 
 def slab_deck(L: float = 4.0, c: float = 0.5, n_bins: int = 100, samples: float = 1e6,
-              seed: int = 1, x0: float = 1e-9) -> str:
-    """One-speed slab deck; with n_bins = 0 the slab is a single cell."""
+              seed: int = 1, x0: float = 1e-9, params=()) -> str:
+    """One-speed slab deck; with n_bins = 0 the slab is a single cell.
+    params: reactions ("scatter", "capture", "total") of gradient parameters
+    declared on the first cell."""
     nb = max(1, n_bins)
     edges = [k * (L / nb) for k in range(nb + 1)]
     edges[-1] = L
@@ -83,7 +85,8 @@ def slab_deck(L: float = 4.0, c: float = 0.5, n_bins: int = 100, samples: float 
         <surface name="px0"/>
     </estimator>
 {flux}</estimators>
-"""
+""" + ("<parameters>\n" + "".join(f'    <parameter cell="bin0" nuclide="onespeed" reaction="{r}"/>\n'
+                                   for r in params) + "</parameters>\n" if params else "")
 
 
 def batched_slabs(n: int = 4096, seed: int = 2024):
@@ -173,9 +176,10 @@ def _shield_common(samples: float, seed: int) -> tuple[str, str]:
     return head, ""
 
 
-def shield_deck(samples: float = 1e8, seed: int = 1, params: str = "") -> str:
+def shield_deck(samples: float = 1e8, seed: int = 1, params: str = "", leakage: bool = True) -> str:
     """Three-layer CE shield.  params: "" or "24" (3 layers x 4 nuclides x
-    {scatter, capture} scale factors)."""
+    {scatter, capture} scale factors).  leakage=False keeps only the
+    transmission estimator."""
     head, _ = _shield_common(samples, seed)
     planes = [0.0, 20.0, 30.0, 60.0]
     surfs = "\n".join(f'    <plane_x name="px{k}" x="{_f(x)}"/>' for k, x in enumerate(planes))
@@ -207,13 +211,16 @@ def shield_deck(samples: float = 1e8, seed: int = 1, params: str = "") -> str:
     <estimator name="transmission" scores="cross">
         <surface name="px3"/>
     </estimator>
-    <estimator name="leakage" scores="cross">
+{LEAKAGE if leakage else ""}</estimators>
+{prm}"""
+
+
+LEAKAGE = """    <estimator name="leakage" scores="cross">
         <surface name="px0"/>
         <surface name="px3"/>
         <filter type="energy" grid_lethargy="1e-5 2e7 100"/>
     </estimator>
-</estimators>
-{prm}"""
+"""
 
 
 # ------------------------------------------------------------------ 3D box
