@@ -227,16 +227,18 @@ LEAKAGE = """    <estimator name="leakage" scores="cross">
 BOX_NX, BOX_NY, BOX_NZ = 5, 2, 2           # 20 shield cells
 BOX_X, BOX_HALF = 60.0, 50.0                # shield 0 <= x <= 60, |y|,|z| <= 50
 WORLD_X = 200.0
-DETECTOR = (160.0, 0.0, 0.0, 10.0)          # sphere centre x, y, z and radius
+DETECTOR = (160.0, 0.0, 0.0, 10.0)          # default: sphere centre x, y, z and radius
 BOX_MATERIAL_OF_COLUMN = ["water", "water", "iron", "concrete", "concrete"]
 
 
-def box_deck(samples: float = 1.5625e7, seed: int = 1, params: bool = True, scales=None) -> str:
+def box_deck(samples: float = 1.5625e7, seed: int = 1, params: bool = True, box_x: float = BOX_X,
+             detector=None) -> str:
     """3D box shield: 20 cells (5 x 2 x 2) of 4 materials filling
-    0 <= x <= 60, |y|, |z| <= 50, then void to x = 200 holding a spherical
-    detector (r = 10 cm) centred 1 m beyond the back face.  Everything outside
+    0 <= x <= box_x (60 cm), |y|, |z| <= 50, then void to x = 200 holding a
+    spherical detector cell (r = 10 cm) centred 1 m beyond the back face;
+    track-length flux in the detector is the response.  Everything outside
     the world box has importance 0.  params: one total-XS scale factor per
-    (cell, nuclide) -> 80 parameters.  scales (20 x 4) perturb the densities."""
+    (cell, nuclide) -> 80 parameters.  detector: (x, y, z, r) override."""
     head, _ = _shield_common(samples, seed)
     # the fourth material is a lighter concrete variant
     head = head.replace("</materials>", '    <material name="light">\n'
@@ -244,7 +246,7 @@ def box_deck(samples: float = 1.5625e7, seed: int = 1, params: bool = True, scal
                         '        <nuclide name="C12x" density="0.002"/>\n'
                         '        <nuclide name="O16x" density="0.030"/>\n'
                         '        <nuclide name="Fe56x" density="0.0002"/>\n    </material>\n</materials>')
-    xs = np.linspace(0.0, BOX_X, BOX_NX + 1)
+    xs = np.linspace(0.0, box_x, BOX_NX + 1)
     ys = np.linspace(-BOX_HALF, BOX_HALF, BOX_NY + 1)
     zs = np.linspace(-BOX_HALF, BOX_HALF, BOX_NZ + 1)
     s = []
@@ -255,7 +257,7 @@ def box_deck(samples: float = 1.5625e7, seed: int = 1, params: bool = True, scal
         s.append(f'    <plane_y name="y{j}" y="{_f(y)}"/>')
     for k, z in enumerate(zs):
         s.append(f'    <plane_z name="z{k}" z="{_f(z)}"/>')
-    cx, cy, cz, r = DETECTOR
+    cx, cy, cz, r = detector if detector is not None else (box_x + 100.0, 0.0, 0.0, 10.0)
     s.append(f'    <sphere name="det" x="{_f(cx)}" y="{_f(cy)}" z="{_f(cz)}" r="{_f(r)}"/>')
     mats = BOX_MATERIAL_OF_COLUMN
     cells = []
