@@ -17,7 +17,10 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmcold.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["mc_xml.cpp", "mc_problem.cpp", "mc_api.cpp", "mc_engine.hip", "mc_kernels.hip"]
+HOST = os.path.join(HERE, "host")
+CLI = os.path.join(HERE, "mc")
+SOURCES = ["csrc/mc_xml.cpp", "csrc/mc_problem.cpp", "csrc/mc_api.cpp", "csrc/mc_engine.hip",
+           "csrc/mc_kernels.hip", "host/simulator.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-I" + CSRC]
 DEVICE = ["--offload-arch=gfx950", "-munsafe-fp-atomics",
@@ -25,8 +28,8 @@ DEVICE = ["--offload-arch=gfx950", "-munsafe-fp-atomics",
 
 
 def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, src + ".o")
-    path = os.path.join(CSRC, src)
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    path = os.path.join(HERE, src)
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE, "-c", path, "-o", obj]
     else:
@@ -47,8 +50,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the `mc <io_dir>` command-line drop-in, linked against the library
+    cmd = ["g++", "-O2", "-std=c++17", "-o", CLI + ".tmp", os.path.join(HOST, "main.cpp"),
+           os.path.join(HOST, "simulator.cpp"), "-L" + HERE, "-lmcold", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", CLI)
     return LIB
 
 

@@ -119,3 +119,17 @@ def test_engine_without_device_fails_cleanly():
     with pytest.raises(mc.MCError) as e:
         mc.Engine(p)
     assert e.value.status == mc.MC_ERR_DEVICE
+
+
+def test_cli_usage_and_missing_device(tmp_path):
+    import subprocess
+    cli = os.path.join(ROOT, "mc-old_amd", "mc")
+    r = subprocess.run([cli], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    d = tmp_path / "deck"
+    d.mkdir()
+    (d / "input.xml").write_text(decks.slab_deck(1.0, 0.5, 2, 100))
+    r = subprocess.run([cli, str(d)], capture_output=True, text=True, timeout=60)
+    import torch
+    if not torch.cuda.is_available():
+        assert r.returncode == 1 and "[ERROR 5]" in r.stderr
