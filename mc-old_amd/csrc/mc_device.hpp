@@ -20,7 +20,7 @@
 
 namespace mc {
 
-constexpr int kBucketBits = 6;                 // buckets per octave = 64
+constexpr int kBucketBits = 8;                 // buckets per octave = 256
 constexpr int kBucketShift = 52 - kBucketBits;
 
 // slot status; flight marks the next event, surface/collision resolve it
@@ -75,6 +75,9 @@ struct KArgs {
     const double* mat_ns;         // per-run scatter density (parameterised materials)
     const double* mat_nc;         // per-run capture density
     const int32_t* mat_param;     // global parameter index
+    const int32_t* mat_grid;      // per material: shared grid of its tables, -1 none, -2 mixed
+    const int32_t* me_tab;        // per material entry: table offset, -1 constant nuclide
+    const double* me_cst;         // per material entry: constant (t, s, c)
     const mc_param_desc* param;   // material/nuc_slot rebased
     const double* theta;
     const DNuc* nuc;
@@ -103,6 +106,7 @@ struct KArgs {
     double *x, *y, *z, *u, *v, *w, *E, *wgt, *sigt;
     uint64_t *rng, *hid;
     int32_t *pcell, *pprob, *psurf;
+    int32_t* pmat;                // material of the particle's cell (-1 void)
     uint32_t* nflight;
     uint8_t* status;
     uint8_t* tcount;              // touched-bin count per slot (kTouched + 1: overflow)
@@ -190,25 +194,49 @@ struct Xs3 {
     double t, s, c;
 };
 
+// Interval of E on material m's shared grid, when all its tabulated nuclides
+// share one (mat_grid[m] >= 0); one search serves every nuclide.
+__device__ __forceinline__ int material_point(const KArgs& A, int m, double E, XsPoint& P)
+{
+    const int g = A.mat_grid[m];
+    if (g >= 0) xs_point(A, g, E, P);
+    return g;
+}
+
+// Microscopic cross section of material entry j (mat_nuclide[j]).  Constant
+// nuclides read me_cst; tabulated ones read their table at the shared
+// interval, or fall back to the nuclide's own grid.
+__device__ __forceinline__ double entry_xs(const KArgs& A, int j, int which, double E, XsPoint& P, int g)
+{
+    const int tb = A.me_tab[j];
+    if (tb < 0) return A.me_cst[3 * j + which];
+    if (g >= 0) {
+        const double* t = which == XS_S ? A.tab_s : which == XS_C ? A.tab_c : A.tab_t;
+        return xs_table(t + tb, P);
+    }
+    return nuclide_xs(A, A.mat_nuc[j], which, E, P);
+}
+
 // Material::SigmaT/S/C (Material.cpp:18-57); parameterised materials use the
 // per-reaction densities and Sigma_t = Sigma_s + Sigma_c.
 __device__ __forceinline__ double material_sigt(const KArgs& A, int m, double E)
 {
     const mc_material_desc M = A.mat[m];
     XsPoint P;
+    const int g = material_point(A, m, E, P);
     if (M.param_count == 0) {
         double t = 0.0;
         for (int k = 0; k < M.nuc_count; ++k) {
             const int j = M.nuc_begin + k;
-            t += nuclide_xs(A, A.mat_nuc[j], XS_T, E, P) * A.mat_dens[j];
+            t += entry_xs(A, j, XS_T, E, P, g) * A.mat_dens[j];
         }
         return t;
     }
     double s = 0.0, c = 0.0;
     for (int k = 0; k < M.nuc_count; ++k) {
         const int j = M.nuc_begin + k;
-        s += nuclide_xs(A, A.mat_nuc[j], XS_S, E, P) * A.mat_ns[j];
-        c += nuclide_xs(A, A.mat_nuc[j], XS_C, E, P) * A.mat_nc[j];
+        s += entry_xs(A, j, XS_S, E, P, g) * A.mat_ns[j];
+        c += entry_xs(A, j, XS_C, E, P, g) * A.mat_nc[j];
     }
     return s + c;
 }
@@ -217,20 +245,21 @@ __device__ __forceinline__ Xs3 material_xs3(const KArgs& A, int m, double E)
 {
     const mc_material_desc M = A.mat[m];
     XsPoint P;
+    const int g = material_point(A, m, E, P);
     Xs3 r = {0.0, 0.0, 0.0};
     if (M.param_count == 0) {
         for (int k = 0; k < M.nuc_count; ++k) {
-            const int j = M.nuc_begin + k, n = A.mat_nuc[j];
+            const int j = M.nuc_begin + k;
             const double N = A.mat_dens[j];
-            r.t += nuclide_xs(A, n, XS_T, E, P) * N;
-            r.s += nuclide_xs(A, n, XS_S, E, P) * N;
-            r.c += nuclide_xs(A, n, XS_C, E, P) * N;
+            r.t += entry_xs(A, j, XS_T, E, P, g) * N;
+            r.s += entry_xs(A, j, XS_S, E, P, g) * N;
+            r.c += entry_xs(A, j, XS_C, E, P, g) * N;
         }
     } else {
         for (int k = 0; k < M.nuc_count; ++k) {
-            const int j = M.nuc_begin + k, n = A.mat_nuc[j];
-            r.s += nuclide_xs(A, n, XS_S, E, P) * A.mat_ns[j];
-            r.c += nuclide_xs(A, n, XS_C, E, P) * A.mat_nc[j];
+            const int j = M.nuc_begin + k;
+            r.s += entry_xs(A, j, XS_S, E, P, g) * A.mat_ns[j];
+            r.c += entry_xs(A, j, XS_C, E, P, g) * A.mat_nc[j];
         }
         r.t = r.s + r.c;
     }
@@ -243,15 +272,16 @@ struct DSig {
     double ds, dc;
 };
 
-__device__ __forceinline__ DSig param_dsig(const KArgs& A, const mc_material_desc& M, int q, double E, XsPoint& P)
+__device__ __forceinline__ DSig param_dsig(const KArgs& A, const mc_material_desc& M, int q, double E, XsPoint& P,
+                                          int g)
 {
     DSig r;
     r.k = A.mat_param[M.param_begin + q];
     const mc_param_desc D = A.param[r.k];
-    const int j = M.nuc_begin + D.nuc_slot, n = A.mat_nuc[j];
+    const int j = M.nuc_begin + D.nuc_slot;
     const double N = A.mat_dens[j];
-    r.ds = (D.reactions & MC_REACTION_SCATTER) ? nuclide_xs(A, n, XS_S, E, P) * N : 0.0;
-    r.dc = (D.reactions & MC_REACTION_CAPTURE) ? nuclide_xs(A, n, XS_C, E, P) * N : 0.0;
+    r.ds = (D.reactions & MC_REACTION_SCATTER) ? entry_xs(A, j, XS_S, E, P, g) * N : 0.0;
+    r.dc = (D.reactions & MC_REACTION_CAPTURE) ? entry_xs(A, j, XS_C, E, P, g) * N : 0.0;
     return r;
 }
 

@@ -27,7 +27,7 @@ __global__ void k_event_scatter(KArgs A);
 __global__ void k_xs_lookup(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
 template <bool kGrad> __global__ void k_surface(KArgs A);
-template <bool kGrad> __global__ void k_collision(KArgs A);
+template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
 __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
                             const int32_t* mats, int32_t nmat, uint64_t seed);
@@ -88,7 +88,8 @@ struct Engine::Impl {
     std::vector<mc_cell_desc> cell;
     std::vector<int32_t> cell_surf, cell_sense, nbr_begin, nbr;
     std::vector<mc_material_desc> mat;
-    std::vector<int32_t> mat_nuc, mat_param;
+    std::vector<int32_t> mat_nuc, mat_param, mat_grid, me_tab;
+    std::vector<double> me_cst;
     std::vector<double> mat_dens;
     std::vector<mc_param_desc> param;
     std::vector<DNuc> nuc;
@@ -110,12 +111,13 @@ struct Engine::Impl {
 
     // device data
     DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
-        d_mat_ns, d_mat_nc, d_mat_param, d_param, d_theta, d_nuc, d_grid, d_grid_E, d_bucket, d_tab_s,
+        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_param, d_theta, d_nuc, d_grid,
+        d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
         d_run_hb;
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
-        b_status, b_tcount, b_tbins, b_hist, b_S, b_G, q_A, q_S, q_C, d_ctr, d_wgcnt;
+        b_status, b_tcount, b_tbins, b_hist, b_S, b_G, q_A, q_S, q_C, d_ctr, d_wgcnt, b_mat;
     uint32_t h_active_cap = 0;
     uint64_t bank_cap = 0;
     bool bank_grad = false;
@@ -313,6 +315,24 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         bin_stride = std::max(bin_stride, d.n_bins);
         par_stride = std::max(par_stride, d.n_params);
     }
+    // per material: the one grid shared by its tabulated nuclides (-1 none,
+    // -2 several); per entry: table offset or the constant values
+    for (size_t m = 0; m < mat.size(); ++m) {
+        int g = -1;
+        for (int k = 0; k < mat[m].nuc_count; ++k) {
+            const DNuc& N = nuc[mat_nuc[mat[m].nuc_begin + k]];
+            if (N.kind != MC_NUC_TABLE) continue;
+            g = (g == -1 || g == N.grid) ? N.grid : -2;
+        }
+        mat_grid.push_back(g);
+    }
+    for (size_t j = 0; j < mat_nuc.size(); ++j) {
+        const DNuc& N = nuc[mat_nuc[j]];
+        me_tab.push_back(N.kind == MC_NUC_TABLE ? N.tab : -1);
+        me_cst.push_back(N.xs_t);     // order t, s, c
+        me_cst.push_back(N.xs_s);
+        me_cst.push_back(N.xs_c);
+    }
     // per surface: cells that list it, in index order
     std::vector<std::vector<int>> lists(surf.size());
     for (size_t c = 0; c < cell.size(); ++c)
@@ -345,6 +365,9 @@ void Engine::Impl::upload_static()
     upload(d_mat_ns, mat_dens, stream);
     upload(d_mat_nc, mat_dens, stream);
     upload(d_mat_param, mat_param, stream);
+    upload(d_mat_grid, mat_grid, stream);
+    upload(d_me_tab, me_tab, stream);
+    upload(d_me_cst, me_cst, stream);
     upload(d_param, param, stream);
     upload(d_theta, std::vector<double>(std::max(1, n_params), 1.0), stream);
     upload(d_nuc, nuc, stream);
@@ -385,6 +408,7 @@ void Engine::Impl::ensure_bank(bool grad)
         b_hid.alloc(8 * B);
         b_cell.alloc(4 * B);
         b_prob.alloc(4 * B);
+        b_mat.alloc(4 * B);
         b_surf.alloc(4 * B);
         b_nfl.alloc(4 * B);
         b_status.alloc(B);
@@ -436,6 +460,9 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.mat_ns = d_mat_ns.as<double>();
     A.mat_nc = d_mat_nc.as<double>();
     A.mat_param = d_mat_param.as<int32_t>();
+    A.mat_grid = d_mat_grid.as<int32_t>();
+    A.me_tab = d_me_tab.as<int32_t>();
+    A.me_cst = d_me_cst.as<double>();
     A.param = d_param.as<mc_param_desc>();
     A.theta = d_theta.as<double>();
     A.nuc = d_nuc.as<DNuc>();
@@ -468,6 +495,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.hid = b_hid.as<uint64_t>();
     A.pcell = b_cell.as<int32_t>();
     A.pprob = b_prob.as<int32_t>();
+    A.pmat = b_mat.as<int32_t>();
     A.psurf = b_surf.as<int32_t>();
     A.nflight = b_nfl.as<uint32_t>();
     A.status = b_status.as<uint8_t>();
@@ -531,8 +559,13 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     else hipLaunchKernelGGL(k_surface<false>, g, b, 0, st, A);
     mark(6, 1);
     mark(7, 0);
-    if (grad) hipLaunchKernelGGL(k_collision<true>, g, b, 0, st, A);
-    else hipLaunchKernelGGL(k_collision<false>, g, b, 0, st, A);
+    if (has_tables) {
+        if (grad) hipLaunchKernelGGL((k_collision<true, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_collision<true, false>), g, b, 0, st, A);
+    } else {
+        if (grad) hipLaunchKernelGGL((k_collision<false, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_collision<false, false>), g, b, 0, st, A);
+    }
     mark(7, 1);
 }
 

@@ -198,8 +198,9 @@ __device__ void score_attachment(const KArgs& A, uint32_t s, int att, double l, 
                 if (m >= 0) {
                     const mc_material_desc M = A.mat[m];
                     XsPoint P;
+                    const int gm = material_point(A, m, E, P);
                     for (int q = 0; q < M.param_count; ++q) {
-                        const DSig d = param_dsig(A, M, q, E, P);
+                        const DSig d = param_dsig(A, M, q, E, P, gm);
                         double dl = 0.0;
                         if (sc == MC_SCORE_SCATTER) dl += d.ds / sx;
                         else if (sc == MC_SCORE_ABSORPTION || sc == MC_SCORE_CAPTURE) dl += d.dc / sx;
@@ -381,6 +382,7 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
             A.rng[i] = rng;
             A.hid[i] = hid;
             A.pcell[i] = S.cell;
+            A.pmat[i] = A.cell[S.cell].material;
             A.pprob[i] = p;
             A.nflight[i] = 0;
             A.tcount[i] = 0;
@@ -400,12 +402,19 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
 
 __global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A)
 {
+    // two independent lookups per thread per step keep more gathers in flight
     const uint32_t n = A.ctr->nA;
-    MC_GRID_LOOP(n)
-    {
-        const uint32_t s = A.qA[_i];
-        const int m = A.cell[A.pcell[s]].material;
-        A.sigt[s] = m >= 0 ? material_sigt(A, m, A.E[s]) : 0.0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+        const uint32_t i2 = i + stride;
+        const uint32_t s = A.qA[i];
+        const uint32_t s2 = i2 < n ? A.qA[i2] : s;
+        const int m = A.pmat[s], m2 = A.pmat[s2];
+        const double E = A.E[s], E2 = A.E[s2];
+        const double r = m >= 0 ? material_sigt(A, m, E) : 0.0;
+        const double r2 = m2 >= 0 ? material_sigt(A, m2, E2) : 0.0;
+        A.sigt[s] = r;
+        if (i2 < n) A.sigt[s2] = r2;
     }
 }
 
@@ -450,8 +459,9 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
             const mc_material_desc M = A.mat[C.material];
             double* Srow = A.S + (size_t)s * (size_t)A.par_stride;
             XsPoint P;
+            const int gm = material_point(A, C.material, E, P);
             for (int q = 0; q < M.param_count; ++q) {
-                const DSig d = param_dsig(A, M, q, E, P);
+                const DSig d = param_dsig(A, M, q, E, P, gm);
                 const double dt = d.ds + d.dc;
                 Srow[d.k - PR.param_begin] += -l * dt + (hit ? 0.0 : dt / sigt);
             }
@@ -593,6 +603,7 @@ __global__ void __launch_bounds__(1024) k_surface(KArgs A)
         A.wgt[s] = w;
         A.rng[s] = rng;
         A.pcell[s] = cell;
+        A.pmat[s] = A.cell[cell].material;
         if (alive) {
             A.status[s] = ST_ALIVE;
         } else {
@@ -602,7 +613,7 @@ __global__ void __launch_bounds__(1024) k_surface(KArgs A)
     }
 }
 
-template <bool kGrad>
+template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_collision(KArgs A)
 {
     const uint32_t n = A.ctr->nC;
@@ -629,29 +640,61 @@ __global__ void __launch_bounds__(1024) k_collision(KArgs A)
                     score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, Srow, Grow);
             }
             const int m = C.material;
-            const Xs3 X = material_xs3(A, m, E);
+            const mc_material_desc M = A.mat[m];
+            const bool scaled = M.param_count != 0;
+            XsPoint P;
+            const int gm = material_point(A, m, E, P);
+            // Sigma_s, Sigma_c (and Sigma_t when no lookup kernel ran) in one pass;
+            // the products sigma_s * N of the first kNucReg entries stay in
+            // registers for the nuclide choice (the same terms Sigma_s sums)
+            constexpr int kNucReg = 8;
+            double ps[kNucReg];
+            Xs3 X = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < kNucReg; ++k) {
+                ps[k] = 0.0;
+                if (k < M.nuc_count) {
+                    const int j = M.nuc_begin + k;
+                    const double Nd = A.mat_dens[j];
+                    ps[k] = entry_xs(A, j, XS_S, E, P, gm) * (scaled ? A.mat_ns[j] : Nd);
+                    X.s += ps[k];
+                    X.c += entry_xs(A, j, XS_C, E, P, gm) * (scaled ? A.mat_nc[j] : Nd);
+                    if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
+                }
+            }
+            for (int k = kNucReg; k < M.nuc_count; ++k) {
+                const int j = M.nuc_begin + k;
+                const double Nd = A.mat_dens[j];
+                X.s += entry_xs(A, j, XS_S, E, P, gm) * (scaled ? A.mat_ns[j] : Nd);
+                X.c += entry_xs(A, j, XS_C, E, P, gm) * (scaled ? A.mat_nc[j] : Nd);
+                if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
+            }
+            // Sigma_t: the lookup kernel's value for this particle is the same sum
+            if (scaled) X.t = X.s + X.c;
+            else if (kTables) X.t = A.sigt[s];
             // implicit capture, general.cpp:154-156 (Sigma_f = 0)
             const double implicit = X.c + 0.0;
             w = w * (X.t - implicit) / X.t;
             // Material::nuclide_scatter, Material.cpp:106-115
-            const mc_material_desc M = A.mat[m];
             const double ur = X.s * urand(rng);
             double acc = 0.0;
             int sel = -1;
-            {
-                XsPoint P;
-                for (int k = 0; k < M.nuc_count; ++k) {
-                    const int j = M.nuc_begin + k;
-                    const double N = M.param_count ? A.mat_ns[j] : A.mat_dens[j];
-                    acc += nuclide_xs(A, A.mat_nuc[j], XS_S, E, P) * N;
-                    if (acc > ur) { sel = k; break; }
+#pragma unroll
+            for (int k = 0; k < kNucReg; ++k) {
+                if (sel < 0 && k < M.nuc_count) {
+                    acc += ps[k];
+                    if (acc > ur) sel = k;
                 }
+            }
+            for (int k = kNucReg; sel < 0 && k < M.nuc_count; ++k) {
+                const int j = M.nuc_begin + k;
+                acc += entry_xs(A, j, XS_S, E, P, gm) * (scaled ? A.mat_ns[j] : A.mat_dens[j]);
+                if (acc > ur) sel = k;
             }
             if (kGrad) {
                 double* Srow = A.S + (size_t)s * (size_t)A.par_stride;
-                XsPoint P;
                 for (int q = 0; q < M.param_count; ++q) {
-                    const DSig d = param_dsig(A, M, q, E, P);
+                    const DSig d = param_dsig(A, M, q, E, P, gm);
                     double v = -(d.ds + d.dc) / X.t;
                     const mc_param_desc D = A.param[d.k];
                     if (sel >= 0 && D.nuc_slot == sel && (D.reactions & MC_REACTION_SCATTER))
@@ -696,9 +739,15 @@ __global__ void __launch_bounds__(1024) k_xs_lookup_array(KArgs A, const double*
                                                          double* __restrict__ out, uint64_t n)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int m = mat[i];
-        out[i] = m >= 0 ? material_sigt(A, m, E[i]) : 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+        const uint64_t i2 = i + stride;
+        const bool two = i2 < n;
+        const int m = mat[i], m2 = two ? mat[i2] : -1;
+        const double e = E[i], e2 = two ? E[i2] : 1.0;
+        const double r = m >= 0 ? material_sigt(A, m, e) : 0.0;
+        const double r2 = m2 >= 0 ? material_sigt(A, m2, e2) : 0.0;
+        out[i] = r;
+        if (two) out[i2] = r2;
     }
 }
 
@@ -718,13 +767,22 @@ __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, d
 }
 
 // out[b] = sum over rows w of part[w * n + b] (rows in order)
+// One block per bin: threads take rows t, t + blockDim, ...; then a fixed
+// tree in LDS (deterministic for a given launch shape).  blockDim <= 256.
 __global__ void k_reduce_rows(const double* part, uint64_t rows, uint64_t n, double* out)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += stride) {
+    __shared__ double red[256];
+    for (uint64_t b = blockIdx.x; b < n; b += gridDim.x) {
         double s = 0.0;
-        for (uint64_t w = 0; w < rows; ++w) s += part[w * n + b];
-        out[b] = s;
+        for (uint64_t w = threadIdx.x; w < rows; w += blockDim.x) s += part[w * n + b];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (unsigned o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[b] = red[0];
+        __syncthreads();
     }
 }
 
@@ -764,7 +822,9 @@ template __global__ void k_flight<false, true>(KArgs);
 template __global__ void k_flight<true, true>(KArgs);
 template __global__ void k_surface<false>(KArgs);
 template __global__ void k_surface<true>(KArgs);
-template __global__ void k_collision<false>(KArgs);
-template __global__ void k_collision<true>(KArgs);
+template __global__ void k_collision<false, false>(KArgs);
+template __global__ void k_collision<true, false>(KArgs);
+template __global__ void k_collision<false, true>(KArgs);
+template __global__ void k_collision<true, true>(KArgs);
 
 }  // namespace mc
