@@ -99,7 +99,7 @@ struct KArgs {
     const uint64_t* run_hb;       // per problem first key of this run (n_prob + 1)
     int32_t n_prob;
     int32_t bin_stride;           // hist row length (max bins per problem)
-    int32_t par_stride;           // S/G row length (max params per problem)
+    int32_t par_stride;           // rows of S and H (max params per problem)
     int32_t pad;
     uint64_t cap;                 // bank capacity
     // bank (SoA)
@@ -107,14 +107,20 @@ struct KArgs {
     uint64_t *rng, *hid;
     int32_t *pcell, *pprob, *psurf;
     int32_t* pmat;                // material of the particle's cell (-1 void)
+    int32_t* gidx;                // interval of E on pmat's shared grid, from the lookup kernel
     uint32_t* nflight;
     uint8_t* status;
     uint8_t* tcount;              // touched-bin count per slot (kTouched + 1: overflow)
     uint32_t* tbins;              // [slot][kTouched] touched bins (problem-local)
     uint32_t* wg_cnt;             // per-workgroup counts for the two-pass compaction
     double* hist;
-    double* S;                    // likelihood-ratio scores [slot][param]
-    double* G;                    // per-history gradient  [slot][param]
+    // likelihood-ratio state: score function S and H = sum of dS * Q, parameter-
+    // major [param][slot] with row stride cap,
+    // and per slot the scores' gradient weight Q, the pending coefficients cs, cc
+    // of dSigma_s/dtheta, dSigma_c/dtheta and the unfolded flight length lpend
+    double* S;
+    double* H;
+    double *Qw, *cs, *cc, *lpend;
     uint32_t *qA, *qS, *qC;
     Counters* ctr;
     // tallies: per-workgroup partial moments [workgroup][bin], reduced at the
@@ -171,6 +177,20 @@ __device__ __forceinline__ void xs_point(const KArgs& A, int gid, double E, XsPo
     }
 }
 
+// The same point from an interval index found earlier at this E.
+__device__ __forceinline__ void xs_point_at(const KArgs& A, int gid, int idx, double E, XsPoint& P)
+{
+    const DGrid g = A.grid[gid];
+    P.grid = gid;
+    P.n = g.n;
+    P.idx = idx;
+    if (idx >= 0 && idx < g.n - 1) {
+        const double E1 = A.grid_E[g.e_off + idx], E2 = A.grid_E[g.e_off + idx + 1];
+        P.a = (E - E2) / (E1 - E2);
+        P.b = (E - E1) / (E2 - E1);
+    }
+}
+
 // XSTable::xs (XSec.cpp:9-40) with the interpolation of Algorithm.cpp:103-105
 __device__ __forceinline__ double xs_table(const double* y, const XsPoint& P)
 {
@@ -203,6 +223,15 @@ __device__ __forceinline__ int material_point(const KArgs& A, int m, double E, X
     return g;
 }
 
+// material_point from the interval the lookup kernel stored for slot s; valid
+// while the particle keeps the material and energy it had at the lookup.
+__device__ __forceinline__ int material_point_cached(const KArgs& A, int m, uint32_t s, double E, XsPoint& P)
+{
+    const int g = A.mat_grid[m];
+    if (g >= 0) xs_point_at(A, g, A.gidx[s], E, P);
+    return g;
+}
+
 // Microscopic cross section of material entry j (mat_nuclide[j]).  Constant
 // nuclides read me_cst; tabulated ones read their table at the shared
 // interval, or fall back to the nuclide's own grid.
@@ -219,11 +248,12 @@ __device__ __forceinline__ double entry_xs(const KArgs& A, int j, int which, dou
 
 // Material::SigmaT/S/C (Material.cpp:18-57); parameterised materials use the
 // per-reaction densities and Sigma_t = Sigma_s + Sigma_c.
-__device__ __forceinline__ double material_sigt(const KArgs& A, int m, double E)
+__device__ __forceinline__ double material_sigt(const KArgs& A, int m, double E, int* idx = nullptr)
 {
     const mc_material_desc M = A.mat[m];
     XsPoint P;
     const int g = material_point(A, m, E, P);
+    if (idx) *idx = P.idx;
     if (M.param_count == 0) {
         double t = 0.0;
         for (int k = 0; k < M.nuc_count; ++k) {

@@ -117,7 +117,7 @@ struct Engine::Impl {
         d_run_hb;
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
-        b_status, b_tcount, b_tbins, b_hist, b_S, b_G, q_A, q_S, q_C, d_ctr, d_wgcnt, b_mat;
+        b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, q_A, q_S, q_C, d_ctr, d_wgcnt, b_mat;
     uint32_t h_active_cap = 0;
     uint64_t bank_cap = 0;
     bool bank_grad = false;
@@ -409,6 +409,7 @@ void Engine::Impl::ensure_bank(bool grad)
         b_cell.alloc(4 * B);
         b_prob.alloc(4 * B);
         b_mat.alloc(4 * B);
+        b_gidx.alloc(4 * B);
         b_surf.alloc(4 * B);
         b_nfl.alloc(4 * B);
         b_status.alloc(B);
@@ -423,7 +424,11 @@ void Engine::Impl::ensure_bank(bool grad)
     }
     if (grad && (!bank_grad || realloc_all)) {
         b_S.alloc(8 * B * (uint64_t)par_stride);
-        b_G.alloc(8 * B * (uint64_t)par_stride);
+        b_H.alloc(8 * B * (uint64_t)par_stride);
+        b_Qw.alloc(8 * B);
+        b_cs.alloc(8 * B);
+        b_cc.alloc(8 * B);
+        b_lp.alloc(8 * B);
         bank_grad = true;
     }
     ++generation;
@@ -496,6 +501,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.pcell = b_cell.as<int32_t>();
     A.pprob = b_prob.as<int32_t>();
     A.pmat = b_mat.as<int32_t>();
+    A.gidx = b_gidx.as<int32_t>();
     A.psurf = b_surf.as<int32_t>();
     A.nflight = b_nfl.as<uint32_t>();
     A.status = b_status.as<uint8_t>();
@@ -504,7 +510,11 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.wg_cnt = d_wgcnt.as<uint32_t>();
     A.hist = b_hist.as<double>();
     A.S = grad ? b_S.as<double>() : nullptr;
-    A.G = grad ? b_G.as<double>() : nullptr;
+    A.H = grad ? b_H.as<double>() : nullptr;
+    A.Qw = grad ? b_Qw.as<double>() : nullptr;
+    A.cs = grad ? b_cs.as<double>() : nullptr;
+    A.cc = grad ? b_cc.as<double>() : nullptr;
+    A.lpend = grad ? b_lp.as<double>() : nullptr;
     A.qA = q_A.as<uint32_t>();
     A.qS = q_S.as<uint32_t>();
     A.qC = q_C.as<uint32_t>();
@@ -686,7 +696,6 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     if (grad) {
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsum, npp, 0.0);
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsq, npp, 0.0);
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.G, bank_cap * (uint64_t)par_stride, 0.0);
     }
     HIP_CHECK(hipEventRecord(ev0, stream));
     const bool plain = run && run->mode == MC_RUN_PLAIN;

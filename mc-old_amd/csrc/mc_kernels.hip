@@ -153,9 +153,15 @@ __device__ __forceinline__ void hist_add(const KArgs& A, uint32_t s, int b, doub
 }
 
 // Estimator::score (Estimator.cpp:298-336) for one attachment.
+// Gradient runs: lp is the flight length not yet folded into S (the track's own
+// length for track-length scores, the pending flight at a collision, 0 after a
+// surface crossing), sigt the cell's Sigma_t at E and coll_end whether the
+// track ends in a collision.  A score of weight gv adds gv to Q and, in a
+// material with parameters, its extra terms to the coefficients cs, cc:
+//   d ln(score)/d theta = S - lp dSigma_t + [track, coll_end] dSigma_t/Sigma_t + d ln Sigma_x.
 template <bool kGrad>
 __device__ void score_attachment(const KArgs& A, uint32_t s, int att, double l, int cellid, double E, double w,
-                                 const DProb& PR, const double* Srow, double* Grow)
+                                 const DProb& PR, double lp, double sigt, bool coll_end)
 {
     const int e = A.att_est[att];
     const mc_estimator_desc& Es = A.est[e];
@@ -194,23 +200,72 @@ __device__ void score_attachment(const KArgs& A, uint32_t s, int att, double l, 
             const double g = A.gout[bin];
             if (g != 0.0 && v != 0.0) {
                 const double gv = g * v;
-                for (int k = 0; k < PR.param_count; ++k) Grow[k] += gv * Srow[k];
-                if (m >= 0) {
-                    const mc_material_desc M = A.mat[m];
-                    XsPoint P;
-                    const int gm = material_point(A, m, E, P);
-                    for (int q = 0; q < M.param_count; ++q) {
-                        const DSig d = param_dsig(A, M, q, E, P, gm);
-                        double dl = 0.0;
-                        if (sc == MC_SCORE_SCATTER) dl += d.ds / sx;
-                        else if (sc == MC_SCORE_ABSORPTION || sc == MC_SCORE_CAPTURE) dl += d.dc / sx;
-                        else if (sc == MC_SCORE_TOTAL) dl += (d.ds + d.dc) / sx;
-                        if (kern == MC_KERNEL_COLLISION) dl -= (d.ds + d.dc) / X.t;
-                        Grow[d.k - PR.param_begin] += gv * dl;
-                    }
+                A.Qw[s] += gv;
+                if (m >= 0 && A.mat[m].param_count > 0) {
+                    double a = -lp;
+                    if (kern == MC_KERNEL_TRACK_LENGTH && coll_end) a += 1.0 / sigt;
+                    double bs = a, bc = a;
+                    if (sc == MC_SCORE_SCATTER) bs += 1.0 / sx;
+                    else if (sc == MC_SCORE_ABSORPTION || sc == MC_SCORE_CAPTURE) bc += 1.0 / sx;
+                    else if (sc == MC_SCORE_TOTAL) { bs += 1.0 / sx; bc += 1.0 / sx; }
+                    A.cs[s] += gv * bs;
+                    A.cc[s] += gv * bc;
                 }
             }
         }
+    }
+}
+
+// Pending likelihood-ratio state of one slot (see KArgs::S).
+struct LrPending {
+    double lp, cs, cc, Q;
+};
+
+__device__ __forceinline__ LrPending lr_pending(const KArgs& A, uint32_t s)
+{
+    return LrPending{A.lpend[s], A.cs[s], A.cc[s], A.Qw[s]};
+}
+
+// Fold one parameter's pending terms into S and H: the scores' coefficients
+// cs, cc of dSigma_s/dtheta, dSigma_c/dtheta, then the increment
+// v = -lp dSigma_t/dtheta + extra of S and its H term v Q.  Terms that are
+// exactly zero are skipped (H never holds -0, so the values are unchanged).
+__device__ __forceinline__ void fold_one(const KArgs& A, uint32_t s, int kl, double ds, double dc, double extra,
+                                         const LrPending& L)
+{
+    // parameter-major rows: lanes of a wave touch neighbouring slots
+    const size_t o = (size_t)kl * A.cap + s;
+    double h = 0.0;
+    const bool hc = L.cs != 0.0 || L.cc != 0.0, hq = L.Q != 0.0;
+    if (hc || hq) h = A.H[o];
+    if (hc) h -= L.cs * ds + L.cc * dc;
+    double v = -L.lp * (ds + dc);
+    v += extra;
+    A.S[o] += v;
+    if (hq) h += v * L.Q;
+    if (hc || hq) A.H[o] = h;
+}
+
+// Surface crossing: fold the flight that ended on the surface, in material M
+// of the cell it leaves.  The parameters' cross sections are gathered first,
+// kChunk at a time, so their loads overlap.
+__device__ void fold_surface(const KArgs& A, uint32_t s, const DProb& PR, const mc_material_desc& M, double E,
+                             XsPoint& P, int gm)
+{
+    constexpr int kChunk = 4;
+    const LrPending L = lr_pending(A, s);
+    for (int q0 = 0; q0 < M.param_count; q0 += kChunk) {
+        DSig d[kChunk];
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t)
+            if (q0 + t < M.param_count) d[t] = param_dsig(A, M, q0 + t, E, P, gm);
+#pragma unroll
+        for (int t = 0; t < kChunk; ++t)
+            if (q0 + t < M.param_count) fold_one(A, s, d[t].k - PR.param_begin, d[t].ds, d[t].dc, 0.0, L);
+    }
+    if (L.cs != 0.0 || L.cc != 0.0) {
+        A.cs[s] = 0.0;
+        A.cc[s] = 0.0;
     }
 }
 
@@ -227,7 +282,8 @@ __device__ __forceinline__ void close_bin(const KArgs& A, double* row, const DPr
 }
 
 // End of a history: close out its tally row (Estimator::end_history).
-template <bool kGrad>
+// kFlush: scores may still be pending (not after a collision, which folds them).
+template <bool kGrad, bool kFlush>
 __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
                                double E, uint32_t nfl)
 {
@@ -269,14 +325,31 @@ __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& P
         }
     }
     if (kGrad) {
-        double* G = A.G + (size_t)s * (size_t)A.par_stride;
+        // scores still pending (surface scores of the cell the particle ends in)
+        double* Srow = A.S + s;
+        double* Hrow = A.H + s;
+        const size_t ks = A.cap;
+        const double Q = A.Qw[s];
+        const double cs = kFlush ? A.cs[s] : 0.0, cc = kFlush ? A.cc[s] : 0.0;
+        if (kFlush && (cs != 0.0 || cc != 0.0)) {
+            const int m = A.pmat[s];
+            if (m >= 0) {
+                const mc_material_desc M = A.mat[m];
+                XsPoint P;
+                const int gm = material_point(A, m, E, P);
+                for (int q = 0; q < M.param_count; ++q) {
+                    const DSig d = param_dsig(A, M, q, E, P, gm);
+                    Hrow[(size_t)(d.k - PR.param_begin) * ks] -= cs * d.ds + cc * d.dc;
+                }
+            }
+        }
+        // the history's gradient sum over its scores gv * S = Q S_end - H
         for (int k = 0; k < PR.param_count; ++k) {
-            const double g = G[k];
+            const double g = Q * Srow[k * ks] - Hrow[k * ks];
             if (g != 0.0) {
                 const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
                 atomicAdd(&A.pgsum[j], g);
                 atomicAdd(&A.pgsq[j], g * g);
-                G[k] = 0.0;
             }
         }
     }
@@ -388,8 +461,14 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
             A.tcount[i] = 0;
             A.status[i] = ST_ALIVE;
             if (kGrad) {
-                double* Srow = A.S + (size_t)i * (size_t)A.par_stride;
-                for (int k = 0; k < PR.param_count; ++k) Srow[k] = 0.0;
+                for (int k = 0; k < PR.param_count; ++k) {
+                    A.S[(size_t)k * A.cap + i] = 0.0;
+                    A.H[(size_t)k * A.cap + i] = 0.0;
+                }
+                A.Qw[i] = 0.0;
+                A.cs[i] = 0.0;
+                A.cc[i] = 0.0;
+                A.lpend[i] = 0.0;
             }
         } else if (need && st != ST_EMPTY) {
             A.status[i] = ST_EMPTY;
@@ -411,10 +490,16 @@ __global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A)
         const uint32_t s2 = i2 < n ? A.qA[i2] : s;
         const int m = A.pmat[s], m2 = A.pmat[s2];
         const double E = A.E[s], E2 = A.E[s2];
-        const double r = m >= 0 ? material_sigt(A, m, E) : 0.0;
-        const double r2 = m2 >= 0 ? material_sigt(A, m2, E2) : 0.0;
+        int g = 0, g2 = 0;
+        const double r = m >= 0 ? material_sigt(A, m, E, &g) : 0.0;
+        const double r2 = m2 >= 0 ? material_sigt(A, m2, E2, &g2) : 0.0;
+        // the interval is kept for the collision and surface kernels (same E)
         A.sigt[s] = r;
-        if (i2 < n) A.sigt[s2] = r2;
+        A.gidx[s] = g;
+        if (i2 < n) {
+            A.sigt[s2] = r2;
+            A.gidx[s2] = g2;
+        }
     }
 }
 
@@ -455,29 +540,18 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         const double l = hit ? dist : dcol;
         const int p_id = A.pprob[s];
         const DProb PR = A.prob[p_id];
-        if (kGrad && C.material >= 0) {
-            const mc_material_desc M = A.mat[C.material];
-            double* Srow = A.S + (size_t)s * (size_t)A.par_stride;
-            XsPoint P;
-            const int gm = material_point(A, C.material, E, P);
-            for (int q = 0; q < M.param_count; ++q) {
-                const DSig d = param_dsig(A, M, q, E, P, gm);
-                const double dt = d.ds + d.dc;
-                Srow[d.k - PR.param_begin] += -l * dt + (hit ? 0.0 : dt / sigt);
-            }
-        }
+        // gradient runs: the flight's -l dSigma_t/dtheta is folded into S at the
+        // collision or surface crossing that ends it, where the per-nuclide
+        // cross sections at E are evaluated anyway
+        if (kGrad && C.material >= 0 && A.mat[C.material].param_count > 0) A.lpend[s] = l;
         // move_particle, general.cpp:73-83
         p.x += u.x * l;
         p.y += u.y * l;
         p.z += u.z * l;
         const uint32_t nfl = A.nflight[s] + 1;
         const double w = A.wgt[s];
-        if (C.tl_count > 0) {
-            const double* Srow = kGrad ? A.S + (size_t)s * (size_t)A.par_stride : nullptr;
-            double* Grow = kGrad ? A.G + (size_t)s * (size_t)A.par_stride : nullptr;
-            for (int a = 0; a < C.tl_count; ++a)
-                score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, Srow, Grow);
-        }
+        for (int a = 0; a < C.tl_count; ++a)
+            score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, l, sigt, !hit);
         A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
         A.rng[s] = rng;
         A.nflight[s] = nfl;
@@ -486,7 +560,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         if (hit && S < 0) {            // no surface ahead: lost particle
             atomicAdd(&A.ctr->err_lost, 1u);
             A.wgt[s] = 0.0;
-            finish_history<kGrad>(A, s, p_id, PR, rng, p, E, nfl);
+            finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, nfl);
             st = ST_DEAD;
         }
         A.status[s] = st;
@@ -556,6 +630,19 @@ __global__ void __launch_bounds__(1024) k_surface(KArgs A)
         const int cell_old = A.pcell[s];
         int cell = cell_old;
         bool alive = true;
+        if (kGrad) {
+            // the flight that ended here, in the cell it leaves
+            const int m = A.cell[cell_old].material;
+            if (m >= 0) {
+                const mc_material_desc M = A.mat[m];
+                if (M.param_count > 0) {
+                    // a shared grid implies tables, so the lookup kernel stored the interval
+                    XsPoint P;
+                    const int gm = material_point_cached(A, m, s, E, P);
+                    fold_surface(A, s, PR, M, E, P, gm);
+                }
+            }
+        }
         // surface_hit, general.cpp:89-115
         if (Sd.bc == MC_BC_TRANSMISSION) {
             p.x += u.x * kEpsFloat;
@@ -577,12 +664,8 @@ __global__ void __launch_bounds__(1024) k_surface(KArgs A)
             p.y += u.y * kEpsFloat;
             p.z += u.z * kEpsFloat;
         }
-        if (Sd.est_count > 0) {
-            const double* Srow = kGrad ? A.S + (size_t)s * (size_t)A.par_stride : nullptr;
-            double* Grow = kGrad ? A.G + (size_t)s * (size_t)A.par_stride : nullptr;
-            for (int a = 0; a < Sd.est_count; ++a)
-                score_attachment<kGrad>(A, s, Sd.est_begin + a, 0.0, cell, E, w, PR, Srow, Grow);
-        }
+        for (int a = 0; a < Sd.est_count; ++a)
+            score_attachment<kGrad>(A, s, Sd.est_begin + a, 0.0, cell, E, w, PR, 0.0, 0.0, false);
         // cell_importance, population_control.cpp:21-49
         const double Iold = A.cell[cell_old].importance;
         const double Inew = A.cell[cell].importance;
@@ -607,7 +690,7 @@ __global__ void __launch_bounds__(1024) k_surface(KArgs A)
         if (alive) {
             A.status[s] = ST_ALIVE;
         } else {
-            finish_history<kGrad>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
+            finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
             A.status[s] = ST_DEAD;
         }
     }
@@ -634,31 +717,40 @@ __global__ void __launch_bounds__(1024) k_collision(KArgs A)
             w = 0.0;
         } else {
             if (C.col_count > 0) {
-                const double* Srow = kGrad ? A.S + (size_t)s * (size_t)A.par_stride : nullptr;
-                double* Grow = kGrad ? A.G + (size_t)s * (size_t)A.par_stride : nullptr;
+                const double lp = kGrad ? A.lpend[s] : 0.0;
                 for (int a = 0; a < C.col_count; ++a)
-                    score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, Srow, Grow);
+                    score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, lp, 0.0, false);
             }
             const int m = C.material;
             const mc_material_desc M = A.mat[m];
             const bool scaled = M.param_count != 0;
             XsPoint P;
-            const int gm = material_point(A, m, E, P);
-            // Sigma_s, Sigma_c (and Sigma_t when no lookup kernel ran) in one pass;
-            // the products sigma_s * N of the first kNucReg entries stay in
-            // registers for the nuclide choice (the same terms Sigma_s sums)
+            const int gm = kTables ? material_point_cached(A, m, s, E, P) : material_point(A, m, E, P);
+            // Sigma_s, Sigma_c (and Sigma_t when no lookup kernel ran) in one pass.
+            // The first kNucReg entries keep in registers what later steps reuse:
+            // the products sigma_s * N for the nuclide choice (the same terms
+            // Sigma_s sums) or, in gradient runs, the microscopic sigma_s, from
+            // which both the choice and dSigma_s/dtheta follow.
             constexpr int kNucReg = 8;
-            double ps[kNucReg];
+            double ps[kGrad ? 1 : kNucReg];
+            double rs[kGrad ? kNucReg : 1];
             Xs3 X = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < kNucReg; ++k) {
-                ps[k] = 0.0;
+                if (!kGrad) ps[k] = 0.0;
                 if (k < M.nuc_count) {
                     const int j = M.nuc_begin + k;
                     const double Nd = A.mat_dens[j];
-                    ps[k] = entry_xs(A, j, XS_S, E, P, gm) * (scaled ? A.mat_ns[j] : Nd);
-                    X.s += ps[k];
-                    X.c += entry_xs(A, j, XS_C, E, P, gm) * (scaled ? A.mat_nc[j] : Nd);
+                    const double xs_s = entry_xs(A, j, XS_S, E, P, gm);
+                    const double xs_c = entry_xs(A, j, XS_C, E, P, gm);
+                    const double prod = xs_s * (scaled ? A.mat_ns[j] : Nd);
+                    if constexpr (kGrad) {
+                        rs[k] = xs_s;
+                    } else {
+                        ps[k] = prod;
+                    }
+                    X.s += prod;
+                    X.c += xs_c * (scaled ? A.mat_nc[j] : Nd);
                     if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
                 }
             }
@@ -682,7 +774,12 @@ __global__ void __launch_bounds__(1024) k_collision(KArgs A)
 #pragma unroll
             for (int k = 0; k < kNucReg; ++k) {
                 if (sel < 0 && k < M.nuc_count) {
-                    acc += ps[k];
+                    if constexpr (kGrad) {
+                        const int j = M.nuc_begin + k;
+                        acc += rs[k] * (scaled ? A.mat_ns[j] : A.mat_dens[j]);
+                    } else {
+                        acc += ps[k];
+                    }
                     if (acc > ur) sel = k;
                 }
             }
@@ -691,15 +788,41 @@ __global__ void __launch_bounds__(1024) k_collision(KArgs A)
                 acc += entry_xs(A, j, XS_S, E, P, gm) * (scaled ? A.mat_ns[j] : A.mat_dens[j]);
                 if (acc > ur) sel = k;
             }
-            if (kGrad) {
-                double* Srow = A.S + (size_t)s * (size_t)A.par_stride;
-                for (int q = 0; q < M.param_count; ++q) {
-                    const DSig d = param_dsig(A, M, q, E, P, gm);
-                    double v = -(d.ds + d.dc) / X.t;
-                    const mc_param_desc D = A.param[d.k];
-                    if (sel >= 0 && D.nuc_slot == sel && (D.reactions & MC_REACTION_SCATTER))
-                        v += 1.0 / A.theta[d.k];
-                    Srow[d.k - PR.param_begin] += v;
+            if constexpr (kGrad) {
+                // Fold the flight that ended here (its 1/Sigma_t collision density
+                // cancels the implicit-capture weight factor's -dSigma_t/Sigma_t)
+                // and the nuclide selection, [scatter off sel]/theta, into S and H.
+                // dSigma_s/dtheta = sigma_s N and dSigma_c/dtheta = sigma_c N of the
+                // parameter's entry: sigma_s from the registers above, sigma_c
+                // read again (the lines are in cache).
+                if (M.param_count > 0) {
+                    const LrPending L = lr_pending(A, s);
+                    for (int q = 0; q < M.param_count; ++q) {
+                        const int kg = A.mat_param[M.param_begin + q];
+                        const mc_param_desc D = A.param[kg];
+                        const int slot = D.nuc_slot, j = M.nuc_begin + slot;
+                        const double N = A.mat_dens[j];
+                        double ds = 0.0, dc = 0.0;
+                        if (D.reactions & MC_REACTION_SCATTER) {
+                            double xs_s = 0.0;
+                            if (slot < kNucReg) {
+#pragma unroll
+                                for (int t = 0; t < kNucReg; ++t)
+                                    if (t == slot) xs_s = rs[t];
+                            } else {
+                                xs_s = entry_xs(A, j, XS_S, E, P, gm);
+                            }
+                            ds = xs_s * N;
+                        }
+                        if (D.reactions & MC_REACTION_CAPTURE) dc = entry_xs(A, j, XS_C, E, P, gm) * N;
+                        const double extra =
+                            (sel >= 0 && slot == sel && (D.reactions & MC_REACTION_SCATTER)) ? 1.0 / A.theta[kg] : 0.0;
+                        fold_one(A, s, kg - PR.param_begin, ds, dc, extra, L);
+                    }
+                    if (L.cs != 0.0 || L.cc != 0.0) {
+                        A.cs[s] = 0.0;
+                        A.cc[s] = 0.0;
+                    }
                 }
             }
             if (sel >= 0) {
@@ -727,7 +850,7 @@ __global__ void __launch_bounds__(1024) k_collision(KArgs A)
             A.status[s] = ST_ALIVE;
         } else {
             const Vec3 p = {A.x[s], A.y[s], A.z[s]};
-            finish_history<kGrad>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
+            finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
             A.status[s] = ST_DEAD;
         }
     }

@@ -403,7 +403,17 @@ typedef struct {
     double E, w;
     int cell, cell_old, surf_old, alive;
     uint64_t rng;
-    double* S;        /* likelihood-ratio score per parameter (gradient runs) */
+    /* likelihood-ratio state (gradient runs).  A score of gradient weight gv
+     * made when the score function is S contributes gv * S to the gradient;
+     * summed over a particle that is Q * S_end - H, with Q the sum of gv and
+     * H the sum over increments dS of dS * (Q when dS was added).  Terms a
+     * score adds on top of S (its own d ln Sigma_x, the pending flight) are
+     * kept as coefficients cs, cc of dSigma_s/dtheta and dSigma_c/dtheta of
+     * the current material at E, folded into H with the pending flight.    */
+    double* S;
+    double* H;
+    double Q, cs, cc;
+    double lpend;     /* length of the last flight, not yet folded into S     */
 } opart;
 
 typedef struct {
@@ -478,8 +488,16 @@ static void add_hist(ostate* st, int bin, double v)
 }
 
 /* Estimator::score, Estimator.cpp:298-336, for one attachment.
- * l: track length (0 for surface and collision events).                  */
-static void estimator_score(ostate* st, const opart* P, int att, double l)
+ * l: track length (0 for surface and collision events); coll_end: the
+ * flight being scored ends in a collision (its length density then carries
+ * Sigma_t, whose log-derivative joins the score's).
+ *
+ * Likelihood-ratio bookkeeping is deferred: a flight of length l leaves
+ * lpend = l instead of adding -l dSigma_t/dtheta to S, and the next
+ * collision or surface crossing folds it in.  A score made before that fold
+ * adds the pending term itself, so every score sees the same total
+ * d ln p / d theta as if S were updated at the flight.                    */
+static void estimator_score(ostate* st, opart* P, int att, double l, int coll_end)
 {
     const mc_problem_desc* d = st->d;
     const int e = d->att_estimator[att];
@@ -498,7 +516,7 @@ static void estimator_score(ostate* st, const opart* P, int att, double l)
     int nds = 0;
     for (int s = 0; s < Es->n_scores; ++s) {
         const int kern = Es->score_kernel[s], sc = Es->score[s];
-        if (!have_x && (kern == MC_KERNEL_COLLISION || sc != MC_SCORE_FLUX)) {
+        if (!have_x && (st->gout || kern == MC_KERNEL_COLLISION || sc != MC_SCORE_FLUX)) {
             if (mat >= 0) X = material_xs3(d, &st->sc, mat, P->E);
             if (st->gout) nds = material_dsig(st, mat, P->E, ds);
             have_x = 1;
@@ -518,16 +536,21 @@ static void estimator_score(ostate* st, const opart* P, int att, double l)
         const int bin = Es->bin_offset + s * Es->n_attach * NE + fi * NE + ebin;
         add_hist(st, bin, v);
         if (st->gout && st->gout[bin] != 0.0 && v != 0.0) {
-            /* d ln(score)/d theta = S + d ln Sigma_x - [collision kernel] d ln Sigma_t */
+            /* d ln(score)/d theta = S - lpend dSigma_t + [track ends in a
+             * collision] dSigma_t/Sigma_t + d ln Sigma_x.  A collision
+             * estimator's 1/Sigma_t cancels the collision density of the
+             * flight that brought the particle here.                        */
             const double gv = st->gout[bin] * v;
-            for (int k = 0; k < st->P; ++k) st->G[k] += gv * P->S[k];
-            for (int q = 0; q < nds; ++q) {
-                double dl = 0.0;
-                if (sc == MC_SCORE_SCATTER) dl += ds[q].ds / sx;
-                else if (sc == MC_SCORE_ABSORPTION || sc == MC_SCORE_CAPTURE) dl += ds[q].dc / sx;
-                else if (sc == MC_SCORE_TOTAL) dl += (ds[q].ds + ds[q].dc) / sx;
-                if (kern == MC_KERNEL_COLLISION) dl -= (ds[q].ds + ds[q].dc) / X.t;
-                st->G[ds[q].k] += gv * dl;
+            P->Q += gv;
+            if (nds > 0) {
+                double a = -P->lpend;
+                if (kern == MC_KERNEL_TRACK_LENGTH && coll_end) a += 1.0 / X.t;
+                double bs = a, bc = a;
+                if (sc == MC_SCORE_SCATTER) bs += 1.0 / sx;
+                else if (sc == MC_SCORE_ABSORPTION || sc == MC_SCORE_CAPTURE) bc += 1.0 / sx;
+                else if (sc == MC_SCORE_TOTAL) { bs += 1.0 / sx; bc += 1.0 / sx; }
+                P->cs += gv * bs;
+                P->cc += gv * bc;
             }
         }
     }
@@ -542,8 +565,14 @@ static void bank_push(ostate* st, const opart* P)
     opart* Q = &st->bank[st->bank_n++];
     *Q = *P;
     if (P->S) {
+        /* the copy carries the score function; the scores made so far stay
+         * with the parent                                                  */
         Q->S = (double*)malloc(sizeof(double) * (size_t)st->P);
         memcpy(Q->S, P->S, sizeof(double) * (size_t)st->P);
+        Q->H = (double*)calloc((size_t)st->P, sizeof(double));
+        Q->Q = 0.0;
+        Q->cs = 0.0;
+        Q->cc = 0.0;
     }
 }
 
@@ -570,11 +599,51 @@ static void cell_importance(ostate* st, opart* P)
     }
 }
 
+/* Fold the pending terms of the current material at E into S and H: the
+ * scores' coefficients cs, cc, then the flight's -lpend dSigma_t/dtheta plus
+ * extra[q] (the nuclide-selection term at a collision).                    */
+static void fold_pending(ostate* st, opart* P, const dsig* ds, int nds, const double* extra)
+{
+    for (int q = 0; q < nds; ++q) {
+        const int k = ds[q].k;
+        P->H[k] -= P->cs * ds[q].ds + P->cc * ds[q].dc;
+        double v = -P->lpend * (ds[q].ds + ds[q].dc);
+        if (extra) v += extra[q];
+        P->S[k] += v;
+        P->H[k] += v * P->Q;
+    }
+    P->lpend = 0.0;
+    P->cs = 0.0;
+    P->cc = 0.0;
+}
+
+/* a flight that ended on a surface, in the cell it leaves */
+static void fold_surface_flight(ostate* st, opart* P)
+{
+    dsig ds[64];
+    const int nds = material_dsig(st, st->d->cells[P->cell].material, P->E, ds);
+    fold_pending(st, P, ds, nds, NULL);
+}
+
+/* end of a particle: scores still pending, then its gradient Q S - H */
+static void close_particle(ostate* st, opart* P)
+{
+    if (P->cs != 0.0 || P->cc != 0.0) {
+        dsig ds[64];
+        const int nds = material_dsig(st, st->d->cells[P->cell].material, P->E, ds);
+        for (int q = 0; q < nds; ++q) P->H[ds[q].k] -= P->cs * ds[q].ds + P->cc * ds[q].dc;
+        P->cs = 0.0;
+        P->cc = 0.0;
+    }
+    for (int k = 0; k < st->P; ++k) st->G[k] += P->Q * P->S[k] - P->H[k];
+}
+
 /* general.cpp:89-115 */
 static void surface_hit(ostate* st, opart* P, int S)
 {
     const mc_problem_desc* d = st->d;
     const mc_surface_desc* Sd = &d->surfaces[S];
+    if (st->gout) fold_surface_flight(st, P);
     P->surf_old = S;
     if (Sd->bc == MC_BC_TRANSMISSION) {
         move(P, EPSILON_FLOAT);
@@ -590,7 +659,7 @@ static void surface_hit(ostate* st, opart* P, int S)
         move(P, EPSILON_FLOAT);
         P->cell_old = P->cell;
     }
-    for (int a = 0; a < Sd->est_count; ++a) estimator_score(st, P, Sd->est_begin + a, 0.0);
+    for (int a = 0; a < Sd->est_count; ++a) estimator_score(st, P, Sd->est_begin + a, 0.0, 0);
     cell_importance(st, P);
 }
 
@@ -619,7 +688,7 @@ static void collision(ostate* st, opart* P, double d_flight)
     const mc_cell_desc* C = &d->cells[P->cell];
     (void)d_flight;
     if (C->material < 0) { kill(P); return; }
-    for (int a = 0; a < C->col_count; ++a) estimator_score(st, P, C->col_begin + a, 0.0);
+    for (int a = 0; a < C->col_count; ++a) estimator_score(st, P, C->col_begin + a, 0.0, 0);
     const int mat = C->material;
     const xs3 X = material_xs3(d, &st->sc, mat, P->E);
     /* implicit capture, general.cpp:154-156 (Sigma_f = 0) */
@@ -637,16 +706,18 @@ static void collision(ostate* st, opart* P, double d_flight)
         if (s > u) { sel = k; break; }
     }
     if (st->gout) {
-        /* weight factor and nuclide selection: -dSigma_t/Sigma_t + [scatter of sel]/theta */
+        /* the flight that ended here contributes -lpend dSigma_t + dSigma_t/Sigma_t,
+         * the implicit-capture weight factor -dSigma_t/Sigma_t, the nuclide
+         * selection [scatter of sel]/theta; the two 1/Sigma_t terms cancel   */
         dsig ds[64];
+        double sel_term[64];
         const int nds = material_dsig(st, mat, P->E, ds);
         for (int q = 0; q < nds; ++q) {
-            double v = -(ds[q].ds + ds[q].dc) / X.t;
             const mc_param_desc* Pd = &d->params[ds[q].k];
-            if (sel >= 0 && Pd->nuc_slot == sel && (Pd->reactions & MC_REACTION_SCATTER))
-                v += 1.0 / (st->theta ? st->theta[ds[q].k] : 1.0);
-            P->S[ds[q].k] += v;
+            sel_term[q] = (sel >= 0 && Pd->nuc_slot == sel && (Pd->reactions & MC_REACTION_SCATTER))
+                              ? 1.0 / (st->theta ? st->theta[ds[q].k] : 1.0) : 0.0;
         }
+        fold_pending(st, P, ds, nds, sel_term);
     }
     if (sel < 0) return;
     scatter(st, P, d->mat_nuclide[M->nuc_begin + sel]);
@@ -677,18 +748,11 @@ static void random_walk(ostate* st, opart* P)
         const int hit = dcol > dist;
         const double l = hit ? dist : dcol;
         if (hit && S < 0) { st->error = MC_ERR_LOST_PARTICLE; kill(P); return; }
-        if (st->gout && C->material >= 0) {
-            dsig ds[64];
-            const int nds = material_dsig(st, C->material, P->E, ds);
-            for (int q = 0; q < nds; ++q) {
-                const double dt = ds[q].ds + ds[q].dc;
-                P->S[ds[q].k] += -l * dt + (hit ? 0.0 : dt / sigt);
-            }
-        }
+        P->lpend = l;   /* folded into S at the collision or surface that ends the flight */
         /* move_particle, general.cpp:73-83 */
         move(P, l);
         st->flights++;
-        for (int a = 0; a < C->tl_count; ++a) estimator_score(st, P, C->tl_begin + a, l);
+        for (int a = 0; a < C->tl_count; ++a) estimator_score(st, P, C->tl_begin + a, l, !hit);
         if (hit) surface_hit(st, P, S);
         else collision(st, P, l);
         /* weight_roulette, population_control.cpp:9-15 */
@@ -698,6 +762,7 @@ static void random_walk(ostate* st, opart* P)
         }
         if (st->error) return;
     }
+    if (st->gout) close_particle(st, P);
 }
 
 /* handler.cpp:11-48 for one cycle of fixed source */
@@ -718,10 +783,11 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
     st.is_touched = (unsigned char*)calloc((size_t)nb, 1);
     st.P = d->n_params;
     st.gout = (grad_out && gsum && gsq && d->n_params > 0) ? grad_out : NULL;
-    double* Sbuf = NULL;
+    double *Sbuf = NULL, *Hbuf = NULL;
     if (st.gout) {
         st.G = (double*)calloc((size_t)st.P, sizeof(double));
         Sbuf = (double*)calloc((size_t)st.P, sizeof(double));
+        Hbuf = (double*)calloc((size_t)st.P, sizeof(double));
     }
     for (uint64_t h = 0; h < n_hist && !st.error; ++h) {
         opart P;
@@ -753,7 +819,12 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
         P.cell = src->cell;
         P.cell_old = src->cell;
         P.surf_old = -1;
-        if (st.gout) { memset(Sbuf, 0, sizeof(double) * (size_t)st.P); P.S = Sbuf; }
+        if (st.gout) {
+            memset(Sbuf, 0, sizeof(double) * (size_t)st.P);
+            memset(Hbuf, 0, sizeof(double) * (size_t)st.P);
+            P.S = Sbuf;
+            P.H = Hbuf;
+        }
         const uint64_t f0 = st.flights;
         random_walk(&st, &P);
         opart last = P;
@@ -761,7 +832,7 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
             opart Q = st.bank[--st.bank_n];
             random_walk(&st, &Q);
             last = Q;
-            if (Q.S) free(Q.S);
+            if (Q.S) { free(Q.S); free(Q.H); }
         }
         /* Estimator::end_history, Estimator.cpp:339-346 */
         double hscore = 0.0;
@@ -795,6 +866,10 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
     if (flights) *flights += st.flights;
     free(st.hist); free(st.touched); free(st.is_touched);
     free(st.sc.ns); free(st.sc.nc);
-    free(st.G); free(Sbuf); free(st.bank);
+    while (st.bank_n > 0) {          /* copies left by an error */
+        opart* Q = &st.bank[--st.bank_n];
+        if (Q->S) { free(Q->S); free(Q->H); }
+    }
+    free(st.G); free(Sbuf); free(Hbuf); free(st.bank);
     return st.error;
 }

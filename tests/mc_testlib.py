@@ -138,6 +138,19 @@ def oracle_run(problem, n: int, key_offset: int = 0, theta=None, trace: int = 0,
     return r
 
 
+REACTION_SCORES = ("flux", "absorption", "scatter", "capture", "total")
+
+
+def reaction_slab(L, c, n):
+    """Single-cell slab with a track-length and a collision estimator of every
+    reaction score on the cell (bins 2-6 and 7-11 after T and R)."""
+    sc = " ".join(REACTION_SCORES)
+    extra = (f'    <estimator name="tl" type="TL" scores="{sc}">\n        <cell name="bin0"/>\n    </estimator>\n'
+             f'    <estimator name="col" type="C" scores="{sc}">\n        <cell name="bin0"/>\n    </estimator>\n')
+    deck = decks.slab_deck(L, c, 0, n, params=("scatter", "capture"))
+    return deck.replace("</estimators>", extra + "</estimators>")
+
+
 _SHIELD_XS = os.path.join(ROOT, "tests", "_xs_cache")
 
 

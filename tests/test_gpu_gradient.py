@@ -3,7 +3,7 @@ the torch.autograd bridge."""
 import numpy as np
 import pytest
 
-from mc_testlib import decks, mc, oracle_run, shield_xs_dir
+from mc_testlib import decks, mc, oracle_run, reaction_slab, shield_xs_dir
 
 pytestmark = pytest.mark.gpu
 
@@ -18,6 +18,20 @@ def test_slab_backward_matches_oracle():
     o = oracle_run(p, n, grad_out=go)
     np.testing.assert_allclose(g.grad, o.grad, rtol=1e-10, atol=1e-14)
     np.testing.assert_allclose(g.grad_sigma, o.grad_sigma, rtol=1e-8)
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12)
+
+
+def test_reaction_score_backward_matches_oracle():
+    """Track-length and collision estimators of flux and every reaction rate
+    (pending score coefficients folded at collisions and surfaces)."""
+    n = 50000
+    p = mc.Problem.from_string(reaction_slab(2.0, 0.6, n))
+    eng = mc.Engine(p, bank_capacity=1 << 14)
+    go = np.random.default_rng(3).normal(size=p.n_bins)
+    theta = np.array([1.07, 0.95])
+    g = eng.backward(go, theta=theta)
+    o = oracle_run(p, n, theta=theta, grad_out=go)
+    np.testing.assert_allclose(g.grad, o.grad, rtol=1e-9, atol=1e-13)
     np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12)
 
 

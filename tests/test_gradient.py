@@ -4,7 +4,8 @@ Pinned by mathematics independent of the Monte Carlo:
   * pure absorber: T = exp(-theta L Sigma_c), dT/dtheta = -L Sigma_c T;
   * one-speed slab: the deterministic slab solution T(tau, c') with optical
     thickness tau = Sigma_t(theta) L and c' = Sigma_s/Sigma_t, differentiated
-    numerically (it is smooth and deterministic);
+    numerically (it is smooth and deterministic); track-length and collision
+    flux and reaction-rate scores follow from absorptions = 1 - T - R;
   * common-random-number central differences of the oracle itself
     (per-history paired differences give their standard error);
   * exact linearity of the vector-Jacobian product in grad_out.
@@ -13,7 +14,7 @@ import numpy as np
 import pytest
 
 import slab_analytic as sa
-from mc_testlib import decks, mc, oracle_run
+from mc_testlib import REACTION_SCORES, decks, mc, oracle_run, reaction_slab
 
 
 def analytic_TR(L, sig_s, sig_c, n=256):
@@ -83,6 +84,62 @@ def test_gradient_vs_common_random_number_differences():
         d = (plus - minus) / (2 * h)
         fd, fd_sig = d.mean(), d.std(ddof=1) / np.sqrt(n)
         assert abs(r.grad[k] - fd) < 3 * np.hypot(r.grad_sigma[k], fd_sig), (k, r.grad[k], fd, fd_sig)
+
+
+def reaction_rates_and_grad(L, c):
+    """Expected flux and reaction rates of the slab and their theta derivatives.
+
+    With a unit source, absorptions A = 1 - T - R; flux = A / Sigma_c and a
+    reaction rate is Sigma_x * flux.  Sigma_s = theta_s c, Sigma_c = theta_c (1 - c)."""
+    T, R = analytic_TR(L, c, 1.0 - c)
+    dTR = analytic_grad(L, c)                       # rows T, R; columns theta_s, theta_c
+    sig_s, sig_c = c, 1.0 - c
+    A = 1.0 - T - R
+    dA = -(dTR[0] + dTR[1])
+    F = A / sig_c
+    dF = np.array([dA[0] / sig_c, (dA[1] - A) / sig_c])
+    dsig = {"s": np.array([sig_s, 0.0]), "c": np.array([0.0, sig_c])}
+    val = {"flux": F, "absorption": A, "capture": A, "scatter": sig_s * F, "total": (sig_s + sig_c) * F}
+    der = {"flux": dF, "absorption": dA, "capture": dA,
+           "scatter": dsig["s"] * F + sig_s * dF,
+           "total": (dsig["s"] + dsig["c"]) * F + (sig_s + sig_c) * dF}
+    return val, der
+
+
+def test_track_length_and_collision_score_gradients_vs_deterministic():
+    """Gradient terms of track-length scores (flight density, d ln Sigma_x) and
+    collision scores (1/Sigma_t kernel) against the deterministic slab."""
+    L, c, n = 2.0, 0.6, 200000
+    p = mc.Problem.from_string(reaction_slab(L, c, n))
+    assert p.n_bins == 12
+    val, der = reaction_rates_and_grad(L, c)
+    for e0 in (2, 7):
+        for j, name in enumerate(REACTION_SCORES):
+            go = np.zeros(p.n_bins)
+            go[e0 + j] = 1.0
+            r = oracle_run(p, n, grad_out=go)
+            assert abs(r.mean[e0 + j] - val[name]) < 4 * r.uncer[e0 + j], (e0, name)
+            z = (r.grad - der[name]) / r.grad_sigma
+            assert np.all(np.abs(z) < 4), (e0, name, r.grad, der[name], r.grad_sigma)
+
+
+def test_gradient_with_importance_splitting_vs_common_random_numbers():
+    """Split copies carry the score function; their scores add to the history's
+    gradient.  Checked against paired central differences of the sum of all
+    bins (T, R and the two cells' track-length flux), the traced history score."""
+    n, h, L, c = 40000, 1e-3, 2.0, 0.6
+    deck = decks.slab_deck(L, c, 2, n, params=("scatter", "capture"))
+    deck = deck.replace('<cell name="bin1" material="slab">', '<cell name="bin1" material="slab" importance="3.0">')
+    p = mc.Problem.from_string(deck)
+    r = oracle_run(p, n, grad_out=np.ones(p.n_bins))
+    for k in range(2):
+        e = np.zeros(2)
+        e[k] = h
+        tp = oracle_run(p, n, theta=1 + e, trace=n).trace
+        tm = oracle_run(p, n, theta=1 - e, trace=n).trace
+        d = (tp["score"] - tm["score"]) / (2 * h)
+        fd, fd_sig = d.mean(), d.std(ddof=1) / np.sqrt(n)
+        assert abs(r.grad[k] - fd) < 3.5 * np.hypot(r.grad_sigma[k], fd_sig), (k, r.grad[k], fd, fd_sig)
 
 
 def test_shield_params_and_materials():
