@@ -184,9 +184,9 @@ typedef struct mc_engine_cfg {
 } mc_engine_cfg;
 
 /* Per-kernel device time of the event loop, recorded when cfg.profile = 1:
- * kernels in order reset, refill_count, refill_scatter, xs_lookup, flight,
- * event_scatter, surface, collision.                                      */
-#define MC_N_KERNELS 8
+ * kernels in order refill_count, refill_scatter, xs_lookup, flight (with the
+ * surface crossings), event_scatter, collision.                            */
+#define MC_N_KERNELS 6
 typedef struct mc_kernel_stats {
     uint64_t launches[MC_N_KERNELS];
     uint64_t items[MC_N_KERNELS];   /* particles (slots for refill) processed  */

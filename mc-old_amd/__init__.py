@@ -57,12 +57,13 @@ class TallyOut(C.Structure):
                 ("flights", C.c_uint64), ("seconds", C.c_double)]
 
 
-KERNEL_NAMES = ("k_reset", "k_refill_count", "k_refill_scatter", "k_xs_lookup", "k_flight",
-                "k_event_scatter", "k_surface", "k_collision")
+KERNEL_NAMES = ("k_refill_count", "k_refill_scatter", "k_xs_lookup", "k_flight", "k_event_scatter",
+                "k_collision")
 
 
 class KernelStats(C.Structure):
-    _fields_ = [("launches", C.c_uint64 * 8), ("items", C.c_uint64 * 8), ("seconds", C.c_double * 8)]
+    _fields_ = [("launches", C.c_uint64 * len(KERNEL_NAMES)), ("items", C.c_uint64 * len(KERNEL_NAMES)),
+                ("seconds", C.c_double * len(KERNEL_NAMES))]
 
 
 class EngineInfo(C.Structure):

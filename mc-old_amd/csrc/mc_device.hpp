@@ -24,7 +24,7 @@ constexpr int kBucketBits = 8;                 // buckets per octave = 256
 constexpr int kBucketShift = 52 - kBucketBits;
 
 // slot status; flight marks the next event, surface/collision resolve it
-enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4 };
+enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_COLL = 4 };
 
 constexpr int kTouched = 8;        // per-slot list of tally bins touched by the history
 
@@ -105,7 +105,7 @@ struct KArgs {
     // bank (SoA)
     double *x, *y, *z, *u, *v, *w, *E, *wgt, *sigt;
     uint64_t *rng, *hid;
-    int32_t *pcell, *pprob, *psurf;
+    int32_t *pcell, *pprob;
     int32_t* pmat;                // material of the particle's cell (-1 void)
     int32_t* gidx;                // interval of E on pmat's shared grid, from the lookup kernel
     uint32_t* nflight;
@@ -121,7 +121,7 @@ struct KArgs {
     double* S;
     double* H;
     double *Qw, *cs, *cc, *lpend;
-    uint32_t *qA, *qS, *qC;
+    uint32_t *qA, *qC;
     Counters* ctr;
     // tallies: per-workgroup partial moments [workgroup][bin], reduced at the
     // end of a run; a row stride of 0 makes every workgroup share one row

@@ -20,13 +20,11 @@
 namespace mc {
 
 // kernels (mc_kernels.hip)
-__global__ void k_reset(Counters* c);
 __global__ void k_refill_count(KArgs A);
 template <bool kGrad> __global__ void k_refill_scatter(KArgs A);
 __global__ void k_event_scatter(KArgs A);
 __global__ void k_xs_lookup(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
-template <bool kGrad> __global__ void k_surface(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
 __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
@@ -116,8 +114,8 @@ struct Engine::Impl {
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
         d_run_hb;
     // bank
-    DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
-        b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, q_A, q_S, q_C, d_ctr, d_wgcnt, b_mat;
+    DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_nfl,
+        b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, q_A, q_C, d_ctr, d_wgcnt, b_mat;
     uint32_t h_active_cap = 0;
     uint64_t bank_cap = 0;
     bool bank_grad = false;
@@ -410,14 +408,12 @@ void Engine::Impl::ensure_bank(bool grad)
         b_prob.alloc(4 * B);
         b_mat.alloc(4 * B);
         b_gidx.alloc(4 * B);
-        b_surf.alloc(4 * B);
         b_nfl.alloc(4 * B);
         b_status.alloc(B);
         b_tcount.alloc(B);
         b_tbins.alloc(4 * (uint64_t)kTouched * B);
         b_hist.alloc(8 * B * (uint64_t)bin_stride);
         q_A.alloc(4 * B);
-        q_S.alloc(4 * B);
         q_C.alloc(4 * B);
         bank_cap = B;
         bank_grad = false;
@@ -502,7 +498,6 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.pprob = b_prob.as<int32_t>();
     A.pmat = b_mat.as<int32_t>();
     A.gidx = b_gidx.as<int32_t>();
-    A.psurf = b_surf.as<int32_t>();
     A.nflight = b_nfl.as<uint32_t>();
     A.status = b_status.as<uint8_t>();
     A.tcount = b_tcount.as<uint8_t>();
@@ -516,7 +511,6 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.cc = grad ? b_cc.as<double>() : nullptr;
     A.lpend = grad ? b_lp.as<double>() : nullptr;
     A.qA = q_A.as<uint32_t>();
-    A.qS = q_S.as<uint32_t>();
     A.qC = q_C.as<uint32_t>();
     A.ctr = d_ctr.as<Counters>();
     A.psum = t_psum.as<double>();
@@ -540,19 +534,16 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
         if (ev) HIP_CHECK(hipEventRecord(ev[2 * k + edge], st));
     };
     mark(0, 0);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, A.ctr);
+    hipLaunchKernelGGL(k_refill_count, g, b, 0, st, A);
     mark(0, 1);
     mark(1, 0);
-    hipLaunchKernelGGL(k_refill_count, g, b, 0, st, A);
-    mark(1, 1);
-    mark(2, 0);
     if (grad) hipLaunchKernelGGL(k_refill_scatter<true>, g, b, 0, st, A);
     else hipLaunchKernelGGL(k_refill_scatter<false>, g, b, 0, st, A);
+    mark(1, 1);
+    mark(2, 0);
+    if (has_tables) hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
     mark(2, 1);
     mark(3, 0);
-    if (has_tables) hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
-    mark(3, 1);
-    mark(4, 0);
     if (has_tables) {
         if (grad) hipLaunchKernelGGL((k_flight<true, true>), g, b, 0, st, A);
         else hipLaunchKernelGGL((k_flight<true, false>), g, b, 0, st, A);
@@ -560,15 +551,11 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
         if (grad) hipLaunchKernelGGL((k_flight<false, true>), g, b, 0, st, A);
         else hipLaunchKernelGGL((k_flight<false, false>), g, b, 0, st, A);
     }
+    mark(3, 1);
+    mark(4, 0);
+    hipLaunchKernelGGL(k_event_scatter, g, b, 0, st, A);
     mark(4, 1);
     mark(5, 0);
-    hipLaunchKernelGGL(k_event_scatter, g, b, 0, st, A);
-    mark(5, 1);
-    mark(6, 0);
-    if (grad) hipLaunchKernelGGL(k_surface<true>, g, b, 0, st, A);
-    else hipLaunchKernelGGL(k_surface<false>, g, b, 0, st, A);
-    mark(6, 1);
-    mark(7, 0);
     if (has_tables) {
         if (grad) hipLaunchKernelGGL((k_collision<true, true>), g, b, 0, st, A);
         else hipLaunchKernelGGL((k_collision<true, false>), g, b, 0, st, A);
@@ -576,7 +563,7 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
         if (grad) hipLaunchKernelGGL((k_collision<false, true>), g, b, 0, st, A);
         else hipLaunchKernelGGL((k_collision<false, false>), g, b, 0, st, A);
     }
-    mark(7, 1);
+    mark(5, 1);
 }
 
 void Engine::Impl::run_loop(bool grad, const KArgs& A)
@@ -617,11 +604,10 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                     float ms = 0.f;
                     HIP_CHECK(hipEventElapsedTime(&ms, kev[2 * j], kev[2 * j + 1]));
                     kstats.seconds[j] += 1e-3 * ms;
-                    kstats.launches[j] += (j == 3 && !has_tables) ? 0 : 1;
+                    kstats.launches[j] += (j == 2 && !has_tables) ? 0 : 1;
                 }
-                kstats.items[0] += 1;
+                kstats.items[0] += h_active_cap;
                 kstats.items[1] += h_active_cap;
-                kstats.items[2] += h_active_cap;
             }
         } else {
             for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream, nullptr);
@@ -636,12 +622,11 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                                          " s (MC_MAX_SECONDS); live particles: " + std::to_string(h_ctr->nA));
     }
     if (profile) {
-        // event counts: flight (and lookup) saw every live particle, then surface / collision
-        kstats.items[3] += has_tables ? h_ctr->tA : 0;
+        // event counts: flight (and lookup) saw every live particle, then collisions
+        kstats.items[2] += has_tables ? h_ctr->tA : 0;
+        kstats.items[3] += h_ctr->tA;
         kstats.items[4] += h_ctr->tA;
-        kstats.items[5] += h_ctr->tA;
-        kstats.items[6] += h_ctr->tS;
-        kstats.items[7] += h_ctr->tC;
+        kstats.items[5] += h_ctr->tC;
     }
 }
 

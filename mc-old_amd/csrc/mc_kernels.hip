@@ -1,21 +1,21 @@
 // mc_kernels.hip — event-based transport kernels for gfx950 (wave64).
 //
 // One event iteration over the structure-of-arrays particle bank:
-//   k_reset           advance the history counter by the last refill's claims
-//   k_refill_count    per-workgroup counts of live and free slots
+//   k_refill_count    per-workgroup counts of live and free slots (and the
+//                     counter roll-over from the previous iteration)
 //   k_refill_scatter  prefix over the counts, then queue A = live slots plus
 //                     new histories started in free slots, in slot order
 //                     (Simulator::start's sample loop, handler.cpp:19-36)
 //   k_xs_lookup       Sigma_t(E) for every particle in A (Material::SigmaT,
 //                     Material.cpp:50-57) — continuous-energy problems only
 //   k_flight          distance to collision vs nearest surface, move, track-
-//                     length tallies, event marking + per-workgroup event counts
-//                     (random_walk + move_particle, general.cpp:40-83, 177-211)
-//   k_event_scatter   prefix over the event counts -> surface queue S and
-//                     collision queue C, in slot order
-//   k_surface         surface crossing: boundary conditions, cell search, surface
-//                     tallies, importance, weight roulette (general.cpp:89-115,
-//                     population_control.cpp:9-49)
+//                     length tallies (random_walk + move_particle, general.cpp:
+//                     40-83, 177-211); a flight that ends on a surface crosses
+//                     it here: boundary conditions, cell search, surface tallies,
+//                     importance, weight roulette (general.cpp:89-115,
+//                     population_control.cpp:9-49); collisions are marked and
+//                     counted per workgroup
+//   k_event_scatter   prefix over the counts -> collision queue C, in slot order
 //   k_collision       collision tallies, implicit capture, nuclide selection,
 //                     elastic scatter, weight roulette (general.cpp:121-163)
 // Queues are built without atomics: wave ballots rank lanes inside a block,
@@ -357,9 +357,14 @@ __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& P
 
 // ---------------------------------------------------------------- kernels
 
-__global__ void k_reset(Counters* c)
+// Pass 1 of the refill: live and free slots per workgroup.  The first thread
+// also rolls the counters over from the previous iteration (no other block
+// reads them before the next kernel).
+__global__ void __launch_bounds__(1024) k_refill_count(KArgs A)
 {
+    __shared__ uint32_t lds[32];
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        Counters* c = A.ctr;
         c->tA += c->nA;            // event counts of the previous iteration
         c->tS += c->nS;
         c->tC += c->nC;
@@ -368,12 +373,6 @@ __global__ void k_reset(Counters* c)
         c->nS = 0;
         c->nC = 0;
     }
-}
-
-// Pass 1 of the refill: live and free slots per workgroup.
-__global__ void __launch_bounds__(1024) k_refill_count(KArgs A)
-{
-    __shared__ uint32_t lds[32];
     uint32_t lo, hi;
     block_range(A.ctr->active_cap, lo, hi);
     uint32_t live = 0, freec = 0;
@@ -503,6 +502,92 @@ __global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A)
     }
 }
 
+// weight_roulette, population_control.cpp:9-15
+__device__ __forceinline__ void weight_roulette(double& w, bool& alive, uint64_t& rng, const DProb& PR)
+{
+    if (w < PR.wr) {
+        if (urand(rng) < w / PR.ws) w = PR.ws;
+        else { alive = false; w = 0.0; }
+    }
+}
+
+// surface_hit, general.cpp:89-115, then cell_importance (population_control.cpp:
+// 21-49) and weight_roulette, for a flight of slot s that ended on surface sf at
+// p.  Writes the particle back and returns its new status.  Kept out of line:
+// about one flight in a hundred crosses a surface in a shield, and the flight
+// kernel's registers stay for the common path.
+template <bool kGrad>
+__device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, int sf, Vec3 p, Vec3 u, double E,
+                                                 double w, uint64_t rng, uint32_t nfl, int cell_old, int p_id)
+{
+    const mc_surface_desc& Sd = A.surf[sf];
+    const DProb PR = A.prob[p_id];
+    int cell = cell_old;
+    bool alive = true;
+    if (kGrad) {
+        // the flight that ended here, in the cell it leaves
+        const int m = A.cell[cell_old].material;
+        if (m >= 0) {
+            const mc_material_desc M = A.mat[m];
+            if (M.param_count > 0) {
+                // a shared grid implies tables, so the lookup kernel stored the interval
+                XsPoint P;
+                const int gm = material_point_cached(A, m, s, E, P);
+                fold_surface(A, s, PR, M, E, P, gm);
+            }
+        }
+    }
+    if (Sd.bc == MC_BC_TRANSMISSION) {
+        p.x += u.x * kEpsFloat;
+        p.y += u.y * kEpsFloat;
+        p.z += u.z * kEpsFloat;
+        cell = find_cell(A, sf, PR, p);
+        if (cell < 0) {
+            atomicAdd(&A.ctr->err_lost, 1u);
+            cell = cell_old;
+            alive = false;
+            w = 0.0;
+        }
+    } else if (Sd.bc == MC_BC_VACUUM) {
+        alive = false;
+        w = 0.0;
+    } else {
+        u = surface_reflect(Sd.type, Sd.c, u);
+        p.x += u.x * kEpsFloat;
+        p.y += u.y * kEpsFloat;
+        p.z += u.z * kEpsFloat;
+    }
+    for (int a = 0; a < Sd.est_count; ++a)
+        score_attachment<kGrad>(A, s, Sd.est_begin + a, 0.0, cell, E, w, PR, 0.0, 0.0, false);
+    // cell_importance, population_control.cpp:21-49
+    const double Iold = A.cell[cell_old].importance;
+    const double Inew = A.cell[cell].importance;
+    if (Inew != Iold) {
+        const double rat = Inew / Iold;
+        if (rat < 1.0) {
+            if (urand(rng) < rat) w = w / rat;
+            else { alive = false; w = 0.0; }
+        } else {
+            const int nsplit = (int)floor(rat + urand(rng));
+            w = w / (double)nsplit;
+            if (nsplit > 1) atomicAdd(&A.ctr->err_split, 1u);   // splitting: unsupported here
+        }
+    }
+    weight_roulette(w, alive, rng, PR);
+    A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+    A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
+    A.wgt[s] = w;
+    A.rng[s] = rng;
+    A.nflight[s] = nfl;
+    A.pcell[s] = cell;
+    A.pmat[s] = A.cell[cell].material;
+    if (alive) return ST_ALIVE;
+    finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, nfl);
+    return ST_DEAD;
+}
+
+// Distance to collision and to the nearest surface, the move and its
+// track-length scores; a flight ending on a surface crosses it here.
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
@@ -552,20 +637,26 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         const double w = A.wgt[s];
         for (int a = 0; a < C.tl_count; ++a)
             score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, l, sigt, !hit);
-        A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-        A.rng[s] = rng;
-        A.nflight[s] = nfl;
-        A.psurf[s] = S;
-        uint8_t st = hit ? ST_SURF : ST_COLL;
-        if (hit && S < 0) {            // no surface ahead: lost particle
+        uint8_t st;
+        if (!hit) {
+            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+            A.rng[s] = rng;
+            A.nflight[s] = nfl;
+            st = ST_COLL;
+            ++ncoll;
+        } else if (S >= 0) {
+            st = surface_crossing<kGrad>(A, s, S, p, u, E, w, rng, nfl, c, p_id);
+            ++nsurf;
+        } else {                        // no surface ahead: lost particle
             atomicAdd(&A.ctr->err_lost, 1u);
+            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+            A.rng[s] = rng;
+            A.nflight[s] = nfl;
             A.wgt[s] = 0.0;
             finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, nfl);
             st = ST_DEAD;
         }
         A.status[s] = st;
-        nsurf += st == ST_SURF;
-        ncoll += st == ST_COLL;
     }
     block_sum2(nsurf, ncoll, lds);
     if (threadIdx.x == 0) {
@@ -574,125 +665,30 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     }
 }
 
-// Surface and collision queues from the flight's marks, in A's order.
+// Collision queue from the flight's marks, in A's order (the flight counted
+// its surface crossings, which it handled itself, beside its collisions).
 __global__ void __launch_bounds__(1024) k_event_scatter(KArgs A)
 {
     __shared__ uint32_t lds[32];
     const uint32_t n = A.ctr->nA;
     uint32_t s_before, c_before, s_tot, c_tot;
     wg_prefix(A.wg_cnt, s_before, c_before, s_tot, c_tot, lds);
+    (void)s_before;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         A.ctr->nS = s_tot;
         A.ctr->nC = c_tot;
     }
     uint32_t lo, hi;
     block_range(n, lo, hi);
-    uint32_t srun = s_before, crun = c_before;
+    uint32_t crun = c_before;
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t s = i < hi ? A.qA[i] : 0u;
         const uint8_t st = i < hi ? A.status[s] : ST_DEAD;
-        uint32_t ns, nc;
-        const uint32_t rs = block_flag_rank(st == ST_SURF, ns, lds);
+        uint32_t nc;
         const uint32_t rc = block_flag_rank(st == ST_COLL, nc, lds);
-        if (st == ST_SURF) A.qS[srun + rs] = s;
         if (st == ST_COLL) A.qC[crun + rc] = s;
-        srun += ns;
         crun += nc;
-    }
-}
-
-// weight_roulette, population_control.cpp:9-15
-__device__ __forceinline__ void weight_roulette(double& w, bool& alive, uint64_t& rng, const DProb& PR)
-{
-    if (w < PR.wr) {
-        if (urand(rng) < w / PR.ws) w = PR.ws;
-        else { alive = false; w = 0.0; }
-    }
-}
-
-template <bool kGrad>
-__global__ void __launch_bounds__(1024) k_surface(KArgs A)
-{
-    const uint32_t n = A.ctr->nS;
-    MC_GRID_LOOP(n)
-    {
-        const uint32_t s = A.qS[_i];
-        const int sf = A.psurf[s];
-        const mc_surface_desc& Sd = A.surf[sf];
-        const int p_id = A.pprob[s];
-        const DProb PR = A.prob[p_id];
-        Vec3 p = {A.x[s], A.y[s], A.z[s]};
-        Vec3 u = {A.u[s], A.v[s], A.w[s]};
-        double w = A.wgt[s];
-        uint64_t rng = A.rng[s];
-        const double E = A.E[s];
-        const int cell_old = A.pcell[s];
-        int cell = cell_old;
-        bool alive = true;
-        if (kGrad) {
-            // the flight that ended here, in the cell it leaves
-            const int m = A.cell[cell_old].material;
-            if (m >= 0) {
-                const mc_material_desc M = A.mat[m];
-                if (M.param_count > 0) {
-                    // a shared grid implies tables, so the lookup kernel stored the interval
-                    XsPoint P;
-                    const int gm = material_point_cached(A, m, s, E, P);
-                    fold_surface(A, s, PR, M, E, P, gm);
-                }
-            }
-        }
-        // surface_hit, general.cpp:89-115
-        if (Sd.bc == MC_BC_TRANSMISSION) {
-            p.x += u.x * kEpsFloat;
-            p.y += u.y * kEpsFloat;
-            p.z += u.z * kEpsFloat;
-            cell = find_cell(A, sf, PR, p);
-            if (cell < 0) {
-                atomicAdd(&A.ctr->err_lost, 1u);
-                cell = cell_old;
-                alive = false;
-                w = 0.0;
-            }
-        } else if (Sd.bc == MC_BC_VACUUM) {
-            alive = false;
-            w = 0.0;
-        } else {
-            u = surface_reflect(Sd.type, Sd.c, u);
-            p.x += u.x * kEpsFloat;
-            p.y += u.y * kEpsFloat;
-            p.z += u.z * kEpsFloat;
-        }
-        for (int a = 0; a < Sd.est_count; ++a)
-            score_attachment<kGrad>(A, s, Sd.est_begin + a, 0.0, cell, E, w, PR, 0.0, 0.0, false);
-        // cell_importance, population_control.cpp:21-49
-        const double Iold = A.cell[cell_old].importance;
-        const double Inew = A.cell[cell].importance;
-        if (Inew != Iold) {
-            const double rat = Inew / Iold;
-            if (rat < 1.0) {
-                if (urand(rng) < rat) w = w / rat;
-                else { alive = false; w = 0.0; }
-            } else {
-                const int nsplit = (int)floor(rat + urand(rng));
-                w = w / (double)nsplit;
-                if (nsplit > 1) atomicAdd(&A.ctr->err_split, 1u);   // splitting: unsupported here
-            }
-        }
-        weight_roulette(w, alive, rng, PR);
-        A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-        A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
-        A.wgt[s] = w;
-        A.rng[s] = rng;
-        A.pcell[s] = cell;
-        A.pmat[s] = A.cell[cell].material;
-        if (alive) {
-            A.status[s] = ST_ALIVE;
-        } else {
-            finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
-            A.status[s] = ST_DEAD;
-        }
     }
 }
 
@@ -943,8 +939,6 @@ template __global__ void k_flight<false, false>(KArgs);
 template __global__ void k_flight<true, false>(KArgs);
 template __global__ void k_flight<false, true>(KArgs);
 template __global__ void k_flight<true, true>(KArgs);
-template __global__ void k_surface<false>(KArgs);
-template __global__ void k_surface<true>(KArgs);
 template __global__ void k_collision<false, false>(KArgs);
 template __global__ void k_collision<true, false>(KArgs);
 template __global__ void k_collision<false, true>(KArgs);
