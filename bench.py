@@ -187,6 +187,12 @@ def main():
         probs = [mc.Problem.from_string(decks.slab_deck(float(l), float(cc), 0, int(k)))
                  for l, cc, k in zip(L, c, N)]
         beng = mc.Engine(probs, use_graph=G)
+        # the autotuner picks workgroup shape, bank capacity and drain threshold
+        # on a 10 % sample (every problem keeps its share of histories)
+        tune = beng.autotune(int(N.sum() // 10))
+        tuned = {"block": tune.block, "grid_per_cu": tune.grid_per_cu, "bank_capacity": int(tune.bank_capacity),
+                 "drain_divisor": tune.drain_divisor, "sample_hist_per_s": tune.histories_per_second}
+        log(f"autotune: {tuned}")
         beng.forward(n_histories=np.maximum(2, N // 1000))   # warm-up (graph capture)
         t = time.perf_counter()
         br = beng.forward()
@@ -194,7 +200,9 @@ def main():
         rate_b = float(N.sum()) / tb
         rng = np.random.default_rng(11)
         sub = np.sort(rng.choice(4096, 64, replace=False))
-        peng2 = mc.Engine([probs[i] for i in sub], use_graph=G)
+        # plain: the same kernels and launch shape, one problem per launch sequence
+        peng2 = mc.Engine([probs[i] for i in sub], use_graph=G, block=tune.block, grid_per_cu=tune.grid_per_cu,
+                          bank_capacity=int(tune.bank_capacity), drain_divisor=tune.drain_divisor)
         peng2.forward(n_histories=np.maximum(2, N[sub] // 1000), mode=mc.MC_RUN_PLAIN)
         t = time.perf_counter()
         peng2.forward(mode=mc.MC_RUN_PLAIN)
@@ -203,7 +211,7 @@ def main():
         out["batched_vs_plain"] = {"problems": 4096, "histories": int(N.sum()), "batched_seconds": tb,
                                    "batched_hist_per_s": rate_b, "plain_subset": int(sub.size),
                                    "plain_seconds": tp, "plain_hist_per_s": rate_p, "speedup": rate_b / rate_p,
-                                   "flights": int(br.flights)}
+                                   "flights": int(br.flights), "autotuned": tuned}
         golden = os.path.join(ROOT, "tests", "golden", "batched_slabs_analytic.npz")
         if os.path.exists(golden):
             g = np.load(golden)

@@ -178,15 +178,19 @@ typedef struct mc_engine_cfg {
     int32_t block;           /* workgroup size (0: 256)                         */
     int32_t grid_per_cu;     /* persistent workgroups per CU (0: 4)             */
     int32_t iters_per_graph; /* event iterations per captured graph (0: 16)     */
-    uint64_t bank_capacity;  /* particle slots (0: 1<<20)                       */
+    uint64_t bank_capacity;  /* particle slots (0: 1<<21)                       */
     int32_t use_graph;       /* 1: replay hipGraphs, 0: plain launches          */
     int32_t profile;         /* 1: plain launches timed per kernel (mc_engine_kernel_stats) */
+    int32_t drain_divisor;   /* once every history has started and at most
+                                bank/drain_divisor particles live, finish them
+                                in one drain launch (0: 1, < 0: never)        */
+    int32_t pad;
 } mc_engine_cfg;
 
 /* Per-kernel device time of the event loop, recorded when cfg.profile = 1:
  * kernels in order refill_count, refill_scatter, xs_lookup, flight (with the
- * surface crossings), event_scatter, collision.                            */
-#define MC_N_KERNELS 6
+ * surface crossings), event_scatter, collision, drain.                     */
+#define MC_N_KERNELS 7
 typedef struct mc_kernel_stats {
     uint64_t launches[MC_N_KERNELS];
     uint64_t items[MC_N_KERNELS];   /* particles (slots for refill) processed  */
@@ -261,12 +265,14 @@ mc_status mc_xs_lookup_bench(mc_engine* e, uint64_t n, int32_t iters, uint64_t s
 mc_status mc_selftest_math(const double* in, double* out, uint64_t n);
 
 typedef struct mc_tune_result {
-    int32_t block, grid_per_cu, iters_per_graph, pad;
+    int32_t block, grid_per_cu, iters_per_graph, drain_divisor;
     uint64_t bank_capacity;
     double histories_per_second;
 } mc_tune_result;
-/* Times short batched runs over a grid of launch shapes on a sample of the
- * engine's histories and installs the fastest.                             */
+/* Times short batched runs over a grid of launch shapes (workgroup size,
+ * workgroups per CU, bank capacity, drain threshold) on a sample of the
+ * engine's histories, every problem keeping its share, and installs the
+ * fastest.  No reference counterpart (the reference has no launch shapes). */
 mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* out);
 
 void mc_engine_destroy(mc_engine* e);

@@ -34,7 +34,8 @@ MC_RUN_BATCHED, MC_RUN_PLAIN = 0, 1
 class EngineCfg(C.Structure):
     _fields_ = [("device", C.c_int32), ("block", C.c_int32), ("grid_per_cu", C.c_int32),
                 ("iters_per_graph", C.c_int32), ("bank_capacity", C.c_uint64),
-                ("use_graph", C.c_int32), ("profile", C.c_int32)]
+                ("use_graph", C.c_int32), ("profile", C.c_int32), ("drain_divisor", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class TraceRecord(C.Structure):
@@ -58,7 +59,7 @@ class TallyOut(C.Structure):
 
 
 KERNEL_NAMES = ("k_refill_count", "k_refill_scatter", "k_xs_lookup", "k_flight", "k_event_scatter",
-                "k_collision")
+                "k_collision", "k_drain")
 
 
 class KernelStats(C.Structure):
@@ -74,7 +75,7 @@ class EngineInfo(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("block", C.c_int32), ("grid_per_cu", C.c_int32), ("iters_per_graph", C.c_int32),
-                ("pad", C.c_int32), ("bank_capacity", C.c_uint64), ("histories_per_second", C.c_double)]
+                ("drain_divisor", C.c_int32), ("bank_capacity", C.c_uint64), ("histories_per_second", C.c_double)]
 
 
 class EstimatorDesc(C.Structure):
@@ -201,7 +202,8 @@ class Problem:
             e += 1
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
+        # at interpreter exit the module globals may already be gone
+        if getattr(self, "_h", None) is not None and self._h.value and lib is not None:
             lib.mc_problem_destroy(self._h)
             self._h = None
 
@@ -226,12 +228,13 @@ class Engine:
 
     def __init__(self, problems, device: int = 0, block: int = 0, grid_per_cu: int = 0,
                  iters_per_graph: int = 0, bank_capacity: int = 0, use_graph: bool = True,
-                 profile: bool = False):
+                 profile: bool = False, drain_divisor: int = 0):
         if isinstance(problems, Problem):
             problems = [problems]
         self.problems = list(problems)
         arr = (C.c_void_p * len(self.problems))(*[p._h.value for p in self.problems])
-        cfg = EngineCfg(device, block, grid_per_cu, iters_per_graph, bank_capacity, int(use_graph), int(profile))
+        cfg = EngineCfg(device, block, grid_per_cu, iters_per_graph, bank_capacity, int(use_graph), int(profile),
+                        int(drain_divisor), 0)
         h = C.c_void_p()
         _check(lib.mc_engine_create(arr, len(self.problems), C.byref(cfg), C.byref(h)))
         self._h = h
@@ -322,7 +325,7 @@ class Engine:
         return r
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
+        if getattr(self, "_h", None) is not None and self._h.value and lib is not None:
             lib.mc_engine_destroy(self._h)
             self._h = None
 

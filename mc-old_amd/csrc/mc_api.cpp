@@ -250,21 +250,23 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
         const mc_engine_cfg base = E.cfg();
         mc_engine_cfg best = base;
         double best_rate = -1.0;
-        const int blocks[] = {128, 256, 512};
-        const int grids[] = {2, 4, 8};
-        const uint64_t banks[] = {1ull << 18, 1ull << 19, 1ull << 20, 1ull << 21};
-        const int iters[] = {8, 32};
+        const int blocks[] = {256, 512};
+        const int grids[] = {2, 4};
+        const uint64_t banks[] = {1ull << 19, 1ull << 20, 1ull << 21};
+        const int drains[] = {1, -1};
         for (uint64_t B : banks) {
             if (B > 4 * sampled && B != banks[0]) continue;
             for (int blk : blocks)
                 for (int gpc : grids)
-                    for (int K : iters) {
+                    for (int dd : drains) {
                         mc_engine_cfg c = base;
                         c.block = blk;
                         c.grid_per_cu = gpc;
                         c.bank_capacity = B;
-                        c.iters_per_graph = K;
+                        c.iters_per_graph = 16;
+                        c.drain_divisor = dd;
                         c.use_graph = 1;
+                        c.profile = 0;
                         E.set_cfg(c);
                         mc_tally_out t{};
                         E.forward(&run, &t);   // warm (graph capture)
@@ -277,6 +279,7 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
         out->block = best.block;
         out->grid_per_cu = best.grid_per_cu;
         out->iters_per_graph = best.iters_per_graph;
+        out->drain_divisor = best.drain_divisor;
         out->bank_capacity = best.bank_capacity;
         out->histories_per_second = best_rate;
     });

@@ -20,11 +20,11 @@
 
 namespace mc {
 
-constexpr int kBucketBits = 8;                 // buckets per octave = 256
+constexpr int kBucketBits = 10;                // buckets per octave = 1024
 constexpr int kBucketShift = 52 - kBucketBits;
 
 // slot status; flight marks the next event, surface/collision resolve it
-enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_COLL = 4 };
+enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4 };
 
 constexpr int kTouched = 8;        // per-slot list of tally bins touched by the history
 
@@ -60,6 +60,32 @@ struct Counters {
     uint64_t tA, tS, tC;          // particles processed by flight / surface / collision
 };
 
+constexpr int kFastNuc = 4;      // nuclides a record's fast path holds in registers
+
+// Interleaved cross-section records of a material whose nuclides are all
+// tabulated on one grid.  Record i of rec_t holds, per nuclide k in material
+// order, (sigma_t[i], sigma_t[i+1]) at 2k; record i of rec_sc holds the pairs
+// of sigma_s at 2k and of sigma_c at c_at + 2k.  They are copies of the nuclide
+// tables, so a lookup reads the same values in one or two cache lines instead
+// of one gather per nuclide and reaction.  The last record repeats its value.
+struct DMatRec {
+    int64_t t_off, sc_off;        // offsets (doubles) of record 0, -1: none
+    int32_t t_stride, sc_stride;  // record sizes in doubles
+    int32_t c_at, pad;            // sigma_c pairs start at c_at: 2 max(n, kFastNuc)
+};
+
+// Per material, everything the fast Sigma_t path reads, in one load: the
+// shared grid (its bucket table holds at most one grid point per bucket) and
+// the material's records (rec_t, or rec_sc for parameterised materials).
+// rec_off < 0: no fast path (more than kFastNuc nuclides, constants, mixed
+// grids or crowded buckets).
+struct DMatLook {
+    int64_t key_base;
+    int64_t rec_off;
+    int32_t e_off, n, b_off, nb;
+    int32_t rec_stride, nuc_count, nuc_begin, scaled;
+};
+
 // Everything a kernel needs, passed by value.
 struct KArgs {
     // problem data
@@ -78,6 +104,10 @@ struct KArgs {
     const int32_t* mat_grid;      // per material: shared grid of its tables, -1 none, -2 mixed
     const int32_t* me_tab;        // per material entry: table offset, -1 constant nuclide
     const double* me_cst;         // per material entry: constant (t, s, c)
+    const DMatRec* mrec;          // per material: interleaved cross-section records
+    const DMatLook* mlook;        // per material: fast Sigma_t path
+    const double* rec_t;
+    const double* rec_sc;
     const mc_param_desc* param;   // material/nuc_slot rebased
     const double* theta;
     const DNuc* nuc;
@@ -105,7 +135,7 @@ struct KArgs {
     // bank (SoA)
     double *x, *y, *z, *u, *v, *w, *E, *wgt, *sigt;
     uint64_t *rng, *hid;
-    int32_t *pcell, *pprob;
+    int32_t *pcell, *pprob, *psurf;
     int32_t* pmat;                // material of the particle's cell (-1 void)
     int32_t* gidx;                // interval of E on pmat's shared grid, from the lookup kernel
     uint32_t* nflight;
@@ -246,14 +276,124 @@ __device__ __forceinline__ double entry_xs(const KArgs& A, int j, int which, dou
     return nuclide_xs(A, A.mat_nuc[j], which, E, P);
 }
 
+// xs_table on a record pair (y[i], y[i+1]) taken at record max(idx, 0).
+__device__ __forceinline__ double xs_pair(const double* r, const XsPoint& P)
+{
+    if (P.idx == P.n - 1 || P.idx == -1) return r[0];
+    return P.a * r[0] + P.b * r[1];
+}
+
+__device__ __forceinline__ const double* rec_at(const double* base, int64_t off, int32_t stride, const XsPoint& P)
+{
+    return base + off + (int64_t)(P.idx > 0 ? P.idx : 0) * stride;
+}
+
+// material_sigt by the fast path: bucket, then the three grid points and the
+// two records the interval can use, all loaded at once, then the choice.  Same
+// interval (binary_search, Algorithm.cpp:46-64), same interpolation and the
+// same sum as material_sigt, so the same bits.  False: m has no fast path.
+__device__ __forceinline__ bool material_sigt_fast(const KArgs& A, int m, double E, double& sig, int& idx)
+{
+    const DMatLook L = A.mlook[m];
+    if (L.rec_off < 0) return false;
+    const int64_t key = (int64_t)(double_to_bits(E) >> kBucketShift) - L.key_base;
+    int i, r0, r1;
+    double E1 = 0.0, E2 = 0.0;
+    bool up = false;
+    if (key < 0) {
+        i = -1; r0 = r1 = 0;
+    } else if (key >= L.nb) {
+        i = L.n - 1; r0 = r1 = L.n - 1;
+    } else {
+        const int lb = A.bucket[L.b_off + key], rb = A.bucket[L.b_off + key + 1];
+        const int left = lb + 1;
+        const double* e = A.grid_E + L.e_off;
+        const double em = e[lb > 0 ? lb : 0];
+        const double e0 = e[left < L.n - 1 ? left : L.n - 1];
+        const double e1 = e[left + 1 < L.n - 1 ? left + 1 : L.n - 1];
+        up = rb >= left && e0 < E;               // the bucket's one point lies below E
+        i = up ? left : lb;
+        E1 = up ? e0 : em;
+        E2 = up ? e1 : e0;
+        r0 = lb > 0 ? lb : 0;
+        r1 = left < L.n - 1 ? left : L.n - 1;
+    }
+    const double* base = L.scaled ? A.rec_sc : A.rec_t;
+    const double* ra = base + L.rec_off + (int64_t)r0 * L.rec_stride;
+    const double* rb2 = base + L.rec_off + (int64_t)r1 * L.rec_stride;
+    // the whole of both candidate records (a record has at least kV doubles
+    // when it serves the fast path), then the one the interval uses
+    constexpr int kV = 4 * kFastNuc;
+    const int cnt = L.scaled ? kV : 2 * kFastNuc;
+    double v[kV];
+#pragma unroll
+    for (int q = 0; q < kV; ++q) {
+        v[q] = 0.0;
+        if (q < cnt) {
+            const double x0 = ra[q], x1 = rb2[q];
+            v[q] = up ? x1 : x0;
+        }
+    }
+    XsPoint P;
+    P.idx = i;
+    P.n = L.n;
+    if (i >= 0 && i < L.n - 1) {
+        P.a = (E - E2) / (E1 - E2);
+        P.b = (E - E1) / (E2 - E1);
+    }
+    idx = i;
+    if (!L.scaled) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < kFastNuc; ++k)
+            if (k < L.nuc_count) t += xs_pair(v + 2 * k, P) * A.mat_dens[L.nuc_begin + k];
+        sig = t;
+        return true;
+    }
+    double s = 0.0, c = 0.0;
+#pragma unroll
+    for (int k = 0; k < kFastNuc; ++k) {
+        if (k < L.nuc_count) {
+            const int j = L.nuc_begin + k;
+            s += xs_pair(v + 2 * k, P) * A.mat_ns[j];
+            c += xs_pair(v + 2 * kFastNuc + 2 * k, P) * A.mat_nc[j];
+        }
+    }
+    sig = s + c;
+    return true;
+}
+
 // Material::SigmaT/S/C (Material.cpp:18-57); parameterised materials use the
 // per-reaction densities and Sigma_t = Sigma_s + Sigma_c.
 __device__ __forceinline__ double material_sigt(const KArgs& A, int m, double E, int* idx = nullptr)
 {
+    double fs;
+    int fi;
+    if (material_sigt_fast(A, m, E, fs, fi)) {
+        if (idx) *idx = fi;
+        return fs;
+    }
     const mc_material_desc M = A.mat[m];
     XsPoint P;
     const int g = material_point(A, m, E, P);
     if (idx) *idx = P.idx;
+    const DMatRec R = A.mrec[m];
+    if (R.t_off >= 0) {
+        if (M.param_count == 0) {
+            const double* r = rec_at(A.rec_t, R.t_off, R.t_stride, P);
+            double t = 0.0;
+            for (int k = 0; k < M.nuc_count; ++k) t += xs_pair(r + 2 * k, P) * A.mat_dens[M.nuc_begin + k];
+            return t;
+        }
+        const double* r = rec_at(A.rec_sc, R.sc_off, R.sc_stride, P);
+        double s = 0.0, c = 0.0;
+        for (int k = 0; k < M.nuc_count; ++k) {
+            const int j = M.nuc_begin + k;
+            s += xs_pair(r + 2 * k, P) * A.mat_ns[j];
+            c += xs_pair(r + R.c_at + 2 * k, P) * A.mat_nc[j];
+        }
+        return s + c;
+    }
     if (M.param_count == 0) {
         double t = 0.0;
         for (int k = 0; k < M.nuc_count; ++k) {

@@ -26,6 +26,8 @@ __global__ void k_event_scatter(KArgs A);
 __global__ void k_xs_lookup(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
+template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
+__global__ void k_drain_close(Counters* c);
 __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
                             const int32_t* mats, int32_t nmat, uint64_t seed);
@@ -88,6 +90,9 @@ struct Engine::Impl {
     std::vector<mc_material_desc> mat;
     std::vector<int32_t> mat_nuc, mat_param, mat_grid, me_tab;
     std::vector<double> me_cst;
+    std::vector<DMatRec> mrec;
+    std::vector<DMatLook> mlook;
+    std::vector<double> rec_t, rec_sc;
     std::vector<double> mat_dens;
     std::vector<mc_param_desc> param;
     std::vector<DNuc> nuc;
@@ -109,12 +114,12 @@ struct Engine::Impl {
 
     // device data
     DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
-        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_param, d_theta, d_nuc, d_grid,
+        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_t, d_rec_sc, d_param, d_theta, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
         d_run_hb;
     // bank
-    DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_nfl,
+    DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
         b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, q_A, q_C, d_ctr, d_wgcnt, b_mat;
     uint32_t h_active_cap = 0;
     uint64_t bank_cap = 0;
@@ -331,6 +336,73 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         me_cst.push_back(N.xs_s);
         me_cst.push_back(N.xs_c);
     }
+    // interleaved records for materials whose nuclides all sit on one grid,
+    // shared by materials with the same nuclide list (cloned cells)
+    std::map<std::vector<int32_t>, DMatRec> rec_ids;
+    for (size_t m = 0; m < mat.size(); ++m) {
+        DMatRec R{-1, -1, 0, 0, 0, 0};
+        const int n = mat[m].nuc_count;
+        bool all_tab = mat_grid[m] >= 0 && n > 0;
+        std::vector<int32_t> key{mat_grid[m]};
+        for (int k = 0; k < n && all_tab; ++k) {
+            const int32_t tb = me_tab[mat[m].nuc_begin + k];
+            all_tab = tb >= 0;
+            key.push_back(tb);
+        }
+        if (all_tab) {
+            auto it = rec_ids.find(key);
+            if (it != rec_ids.end()) {
+                R = it->second;
+            } else {
+                const DGrid& G = grid[mat_grid[m]];
+                R.t_stride = (2 * n + 7) / 8 * 8;           // whole 64-byte units
+                R.c_at = 2 * std::max(n, kFastNuc);
+                R.sc_stride = (R.c_at + 2 * n + 7) / 8 * 8;
+                R.t_off = (int64_t)rec_t.size();
+                R.sc_off = (int64_t)rec_sc.size();
+                rec_t.resize(rec_t.size() + (size_t)G.n * R.t_stride, 0.0);
+                rec_sc.resize(rec_sc.size() + (size_t)G.n * R.sc_stride, 0.0);
+                for (int i = 0; i < G.n; ++i) {
+                    const int i2 = i + 1 < G.n ? i + 1 : i;
+                    double* rt = rec_t.data() + R.t_off + (int64_t)i * R.t_stride;
+                    double* rs = rec_sc.data() + R.sc_off + (int64_t)i * R.sc_stride;
+                    for (int k = 0; k < n; ++k) {
+                        const int tb = me_tab[mat[m].nuc_begin + k];
+                        rt[2 * k] = tab_t[tb + i];
+                        rt[2 * k + 1] = tab_t[tb + i2];
+                        rs[2 * k] = tab_s[tb + i];
+                        rs[2 * k + 1] = tab_s[tb + i2];
+                        rs[R.c_at + 2 * k] = tab_c[tb + i];
+                        rs[R.c_at + 2 * k + 1] = tab_c[tb + i2];
+                    }
+                }
+                rec_ids.emplace(key, R);
+            }
+        }
+        mrec.push_back(R);
+        // fast Sigma_t path: records, at most kFastNuc nuclides, one point per bucket
+        DMatLook L{};
+        L.rec_off = -1;
+        if (R.t_off >= 0 && n <= kFastNuc) {
+            const DGrid& G = grid[mat_grid[m]];
+            int crowd = 0;
+            for (int k = 0; k < G.nb; ++k) crowd = std::max(crowd, bucket[G.b_off + k + 1] - bucket[G.b_off + k]);
+            if (crowd <= 1) {
+                L.key_base = G.key_base;
+                L.scaled = mat[m].param_count > 0;
+                L.rec_off = L.scaled ? R.sc_off : R.t_off;
+                L.rec_stride = L.scaled ? R.sc_stride : R.t_stride;
+                L.e_off = G.e_off;
+                L.n = G.n;
+                L.b_off = G.b_off;
+                L.nb = G.nb;
+                L.nuc_count = n;
+                L.nuc_begin = mat[m].nuc_begin;
+            }
+        }
+        mlook.push_back(L);
+    }
+    if (rec_t.empty()) { rec_t.push_back(0.0); rec_sc.push_back(0.0); }
     // per surface: cells that list it, in index order
     std::vector<std::vector<int>> lists(surf.size());
     for (size_t c = 0; c < cell.size(); ++c)
@@ -366,6 +438,10 @@ void Engine::Impl::upload_static()
     upload(d_mat_grid, mat_grid, stream);
     upload(d_me_tab, me_tab, stream);
     upload(d_me_cst, me_cst, stream);
+    upload(d_mrec, mrec, stream);
+    upload(d_mlook, mlook, stream);
+    upload(d_rec_t, rec_t, stream);
+    upload(d_rec_sc, rec_sc, stream);
     upload(d_param, param, stream);
     upload(d_theta, std::vector<double>(std::max(1, n_params), 1.0), stream);
     upload(d_nuc, nuc, stream);
@@ -407,6 +483,7 @@ void Engine::Impl::ensure_bank(bool grad)
         b_cell.alloc(4 * B);
         b_prob.alloc(4 * B);
         b_mat.alloc(4 * B);
+        b_surf.alloc(4 * B);
         b_gidx.alloc(4 * B);
         b_nfl.alloc(4 * B);
         b_status.alloc(B);
@@ -464,6 +541,10 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.mat_grid = d_mat_grid.as<int32_t>();
     A.me_tab = d_me_tab.as<int32_t>();
     A.me_cst = d_me_cst.as<double>();
+    A.mrec = d_mrec.as<DMatRec>();
+    A.mlook = d_mlook.as<DMatLook>();
+    A.rec_t = d_rec_t.as<double>();
+    A.rec_sc = d_rec_sc.as<double>();
     A.param = d_param.as<mc_param_desc>();
     A.theta = d_theta.as<double>();
     A.nuc = d_nuc.as<DNuc>();
@@ -497,6 +578,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.pcell = b_cell.as<int32_t>();
     A.pprob = b_prob.as<int32_t>();
     A.pmat = b_mat.as<int32_t>();
+    A.psurf = b_surf.as<int32_t>();
     A.gidx = b_gidx.as<int32_t>();
     A.nflight = b_nfl.as<uint32_t>();
     A.status = b_status.as<uint8_t>();
@@ -524,6 +606,9 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.trace_count = trace_count;
     return A;
 }
+
+// Kernels of one event iteration (the first entries of mc_kernel_stats).
+constexpr int kIterKernels = 6;
 
 // One event iteration.  With ev (2 * MC_N_KERNELS events) each kernel is
 // bracketed by a start/stop event pair for per-kernel timing.
@@ -593,14 +678,66 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
     }
     const auto t0 = std::chrono::steady_clock::now();
     const double max_s = getenv("MC_MAX_SECONDS") ? atof(getenv("MC_MAX_SECONDS")) : 3600.0;
+    const dim3 g = grid_dim(), b((unsigned)cfg.block);
+    uint64_t drain_flights = 0, drain_coll = 0;
+    auto refill = [&]() {
+        hipLaunchKernelGGL(k_refill_count, g, b, 0, stream, A);
+        if (grad) hipLaunchKernelGGL(k_refill_scatter<true>, g, b, 0, stream, A);
+        else hipLaunchKernelGGL(k_refill_scatter<false>, g, b, 0, stream, A);
+    };
+    auto read_ctr = [&]() {
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    };
     for (uint64_t chunk = 0;; ++chunk) {
+        // The tail of a run: every history has started and few particles are
+        // left, so an iteration would mostly pay launch latencies.  One drain
+        // launch carries them to the end of their histories instead.
+        const bool drain = chunk > 0 && cfg.drain_divisor > 0 &&
+                           h_ctr->next_hist + h_ctr->claimed >= h_ctr->end_hist &&
+                           (uint64_t)h_ctr->nA * (uint64_t)cfg.drain_divisor <= (uint64_t)h_active_cap;
+        if (drain) {
+            // the roll-over in the refill below adds the last iteration's counts
+            const uint64_t tA0 = h_ctr->tA + h_ctr->nA, tC0 = h_ctr->tC + h_ctr->nC;
+            refill();
+            if (profile) HIP_CHECK(hipEventRecord(kev[12], stream));
+            const uint32_t kMaxEvents = 1u << 14;
+            if (has_tables) {
+                if (grad) hipLaunchKernelGGL((k_drain<true, true>), g, b, 0, stream, A, kMaxEvents);
+                else hipLaunchKernelGGL((k_drain<true, false>), g, b, 0, stream, A, kMaxEvents);
+            } else {
+                if (grad) hipLaunchKernelGGL((k_drain<false, true>), g, b, 0, stream, A, kMaxEvents);
+                else hipLaunchKernelGGL((k_drain<false, false>), g, b, 0, stream, A, kMaxEvents);
+            }
+            if (profile) HIP_CHECK(hipEventRecord(kev[13], stream));
+            // the drain counted its own events; the refill after it must not
+            // count the drained particles again
+            hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, stream, A.ctr);
+            refill();
+            read_ctr();
+            drain_flights += h_ctr->tA - tA0;
+            drain_coll += h_ctr->tC - tC0;
+            if (profile) {
+                float ms = 0.f;
+                HIP_CHECK(hipEventSynchronize(kev[13]));
+                HIP_CHECK(hipEventElapsedTime(&ms, kev[12], kev[13]));
+                kstats.seconds[6] += 1e-3 * ms;
+                kstats.launches[6] += 1;
+            }
+            if (h_ctr->nA == 0) break;
+            // histories longer than the drain's event limit: back to iterations,
+            // whose first roll-over must not count this refill's particles
+            hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, stream, A.ctr);
+            continue;
+        }
         if (use_graph) {
             HIP_CHECK(hipGraphLaunch(graph[gi], stream));
         } else if (profile) {
             for (int k = 0; k < K; ++k) {
                 launch_iteration(A, grad, stream, kev.data());
-                HIP_CHECK(hipEventSynchronize(kev.back()));
-                for (int j = 0; j < MC_N_KERNELS; ++j) {
+                HIP_CHECK(hipEventSynchronize(kev[2 * kIterKernels - 1]));
+                for (int j = 0; j < kIterKernels; ++j) {
                     float ms = 0.f;
                     HIP_CHECK(hipEventElapsedTime(&ms, kev[2 * j], kev[2 * j + 1]));
                     kstats.seconds[j] += 1e-3 * ms;
@@ -612,9 +749,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
         } else {
             for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream, nullptr);
         }
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
+        read_ctr();
         if (h_ctr->nA == 0) break;
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (el > max_s)
@@ -622,11 +757,14 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                                          " s (MC_MAX_SECONDS); live particles: " + std::to_string(h_ctr->nA));
     }
     if (profile) {
-        // event counts: flight (and lookup) saw every live particle, then collisions
-        kstats.items[2] += has_tables ? h_ctr->tA : 0;
-        kstats.items[3] += h_ctr->tA;
-        kstats.items[4] += h_ctr->tA;
-        kstats.items[5] += h_ctr->tC;
+        // event counts: flight (and lookup) saw every live particle of the
+        // iterations, then collisions; the drain's flights are its own
+        const uint64_t it_flights = h_ctr->tA - drain_flights;
+        kstats.items[2] += has_tables ? it_flights : 0;
+        kstats.items[3] += it_flights;
+        kstats.items[4] += it_flights;
+        kstats.items[5] += h_ctr->tC - drain_coll;
+        kstats.items[6] += drain_flights;
     }
 }
 
@@ -765,7 +903,8 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     if (cfg.block == 0) cfg.block = 256;
     if (cfg.grid_per_cu == 0) cfg.grid_per_cu = 4;
     if (cfg.iters_per_graph == 0) cfg.iters_per_graph = 16;
-    if (cfg.bank_capacity == 0) cfg.bank_capacity = 1ull << 20;
+    if (cfg.bank_capacity == 0) cfg.bank_capacity = 1ull << 21;
+    if (cfg.drain_divisor == 0) cfg.drain_divisor = 1;
     if (cfg.block % 64 || cfg.block > 1024) throw Error(MC_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
     if (cfg.bank_capacity > (1ull << 31)) throw Error(MC_ERR_CAPACITY, "bank capacity above 2^31");
     d->cfg = cfg;
@@ -893,6 +1032,7 @@ void Engine::set_cfg(const mc_engine_cfg& c)
     d->cfg.grid_per_cu = c.grid_per_cu;
     d->cfg.iters_per_graph = c.iters_per_graph;
     d->cfg.use_graph = c.use_graph;
+    d->cfg.drain_divisor = c.drain_divisor == 0 ? 1 : c.drain_divisor;
     if (c.bank_capacity != d->cfg.bank_capacity) d->cfg.bank_capacity = c.bank_capacity;
     ++d->generation;
 }

@@ -388,11 +388,95 @@ __global__ void __launch_bounds__(1024) k_refill_count(KArgs A)
     }
 }
 
+// A new history in slot i: SourceBank::get_source + SourcePoint::get_source
+// (Source.cpp:20-45).  jstate is the LCG multiplier g^(stride * hid) of its
+// key, so the stream starts where RN_init_particle (Random.cpp:196-204) puts it.
+template <bool kGrad>
+__device__ __forceinline__ void start_history(const KArgs& A, uint32_t i, uint64_t hid, int p, uint64_t jstate)
+{
+    const DProb PR = A.prob[p];
+    uint64_t rng = (jstate * PR.seed) & kRnMask;
+    const double xi = urand(rng);
+    int si = binary_search(xi, A.src_cdf + A.cdf_off[p], PR.src_count + 1);
+    si = si < 0 ? 0 : (si >= PR.src_count ? PR.src_count - 1 : si);   // cdf[n] may round below 1
+    const mc_source_desc& S = A.src[PR.src_begin + si];
+    double ux, uy, uz;
+    if (S.dir_kind == MC_DIR_DELTA) {
+        ux = S.dir[0]; uy = S.dir[1]; uz = S.dir[2];
+    } else {                                        // Distribution.cpp:78-92
+        const double mu = 2.0 * urand(rng) - 1.0;
+        double ca, sa;
+        sincos_2pi(urand(rng), ca, sa);
+        const double c = sqrt(1.0 - mu * mu);
+        uy = ca * c;
+        uz = sa * c;
+        ux = mu;
+    }
+    double E0;
+    if (S.energy_kind == MC_ENERGY_UNIFORM)          // Distribution.cpp:30-33
+        E0 = S.energy[0] + urand(rng) * (S.energy[1] - S.energy[0]);
+    else
+        E0 = S.energy[0];
+    A.x[i] = S.pos[0]; A.y[i] = S.pos[1]; A.z[i] = S.pos[2];
+    A.u[i] = ux; A.v[i] = uy; A.w[i] = uz;
+    A.E[i] = E0;
+    A.wgt[i] = 1.0;
+    A.rng[i] = rng;
+    A.hid[i] = hid;
+    A.pcell[i] = S.cell;
+    A.pmat[i] = A.cell[S.cell].material;
+    A.pprob[i] = p;
+    A.nflight[i] = 0;
+    A.tcount[i] = 0;
+    A.status[i] = ST_ALIVE;
+    if (kGrad) {
+        for (int k = 0; k < PR.param_count; ++k) {
+            A.S[(size_t)k * A.cap + i] = 0.0;
+            A.H[(size_t)k * A.cap + i] = 0.0;
+        }
+        A.Qw[i] = 0.0;
+        A.cs[i] = 0.0;
+        A.cc[i] = 0.0;
+        A.lpend[i] = 0.0;
+    }
+}
+
+// Block-wide exclusive prefix of two counts per thread (wave shuffles, then
+// the wave totals through LDS); also returns the block totals.
+__device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& a_ex, uint32_t& b_ex, uint32_t& a_tot,
+                                            uint32_t& b_tot, uint32_t* lds)
+{
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t ia = a, ib = b;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t ta = __shfl_up(ia, o), tb = __shfl_up(ib, o);
+        if (lane >= (uint32_t)o) { ia += ta; ib += tb; }
+    }
+    if (lane == 63) { lds[2 * wave] = ia; lds[2 * wave + 1] = ib; }
+    __syncthreads();
+    uint32_t wa = 0, wb = 0, ta = 0, tb = 0;
+    for (uint32_t k = 0; k < nw; ++k) {
+        const uint32_t x = lds[2 * k], y = lds[2 * k + 1];
+        if (k < wave) { wa += x; wb += y; }
+        ta += x;
+        tb += y;
+    }
+    __syncthreads();
+    a_ex = wa + ia - a;
+    b_ex = wb + ib - b;
+    a_tot = ta;
+    b_tot = tb;
+}
+
 // Pass 2 of the refill: slot-ordered queue A and new histories, no atomics.
+// Each thread owns a run of consecutive slots of its block's range, so one
+// block scan places every thread's live slots and new histories.  A wave
+// finds the problem and the RN stream multiplier once for its first new key;
+// the other keys of the wave are a short step from it.
 template <bool kGrad>
 __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
 {
-    __shared__ uint32_t lds[32];
+    __shared__ uint32_t lds[64];
     const uint32_t cap = A.ctr->active_cap;
     const uint64_t next = A.ctr->next_hist, end = A.ctr->end_hist;
     const uint64_t remaining = end > next ? end - next : 0;
@@ -405,76 +489,46 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
     }
     uint32_t lo, hi;
     block_range(cap, lo, hi);
-    // slots of this block that are free and get a history: free rank < remaining
-    uint64_t free_run = free_before;               // free slots before the current tile
-    uint64_t act_run = live_before + (free_before < remaining ? free_before : remaining);
-    for (uint32_t base = lo; base < hi; base += blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        const bool valid = i < hi;
-        const uint8_t st = valid ? A.status[i] : ST_ALIVE;
-        const bool keep = valid && st == ST_ALIVE;
-        const bool need = valid && (st == ST_DEAD || st == ST_EMPTY);
-        uint32_t nfree;
-        const uint32_t frank = block_flag_rank(need, nfree, lds);
-        const uint64_t fidx = free_run + frank;
-        const bool got = need && fidx < remaining;
-        uint32_t nact;
-        const uint32_t arank = block_flag_rank(keep || got, nact, lds);
-        if (got) {
-            // SourceBank::get_source + SourcePoint::get_source (Source.cpp:20-45)
-            const uint64_t hid = next + fidx;
-            const int p = find_problem(A, hid);
-            const DProb PR = A.prob[p];
-            uint64_t rng = history_seed(PR.seed, hid, A.jump);
-            const double xi = urand(rng);
-            int si = binary_search(xi, A.src_cdf + A.cdf_off[p], PR.src_count + 1);
-            si = si < 0 ? 0 : (si >= PR.src_count ? PR.src_count - 1 : si);   // cdf[n] may round below 1
-            const mc_source_desc& S = A.src[PR.src_begin + si];
-            double ux, uy, uz;
-            if (S.dir_kind == MC_DIR_DELTA) {
-                ux = S.dir[0]; uy = S.dir[1]; uz = S.dir[2];
-            } else {                                        // Distribution.cpp:78-92
-                const double mu = 2.0 * urand(rng) - 1.0;
-                double ca, sa;
-                sincos_2pi(urand(rng), ca, sa);
-                const double c = sqrt(1.0 - mu * mu);
-                uy = ca * c;
-                uz = sa * c;
-                ux = mu;
+    const uint32_t per = (hi - lo + blockDim.x - 1) / blockDim.x;
+    const uint32_t a = min(hi, lo + threadIdx.x * per), b = min(hi, a + per);
+    uint32_t nlive = 0, nfree = 0;
+    for (uint32_t i = a; i < b; ++i) {
+        const uint8_t st = A.status[i];
+        nlive += st == ST_ALIVE;
+        nfree += st == ST_DEAD || st == ST_EMPTY;
+    }
+    uint32_t live_ex, free_ex, lt, ft;
+    block_scan2(nlive, nfree, live_ex, free_ex, lt, ft, lds);
+    uint64_t fidx = (uint64_t)free_before + free_ex;          // rank of my next free slot
+    uint64_t act = (uint64_t)live_before + live_ex + (fidx < remaining ? fidx : remaining);
+    // the wave's first new key: the smallest fidx of a lane that starts one
+    const bool starts = nfree > 0 && fidx < remaining;
+    const uint64_t mask = __ballot(starts);
+    uint64_t h0 = 0, j0 = 1;
+    int p0 = 0;
+    if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        h0 = next + (uint64_t)__shfl((unsigned long long)fidx, leader);
+        p0 = find_problem(A, h0);
+        j0 = history_seed(1, h0, A.jump);              // g^(stride * h0)
+    }
+    int p = p0;
+    for (uint32_t i = a; i < b; ++i) {
+        const uint8_t st = A.status[i];
+        if (st == ST_ALIVE) {
+            A.qA[act++] = i;
+        } else if (st == ST_DEAD || st == ST_EMPTY) {
+            if (fidx < remaining) {
+                const uint64_t hid = next + fidx;
+                while (p + 1 < A.n_prob && A.run_hb[p + 1] <= hid) ++p;
+                const uint64_t js = history_seed(j0, hid - h0, A.jump);
+                start_history<kGrad>(A, i, hid, p, js);
+                A.qA[act++] = i;
+            } else if (st != ST_EMPTY) {
+                A.status[i] = ST_EMPTY;
             }
-            double E0;
-            if (S.energy_kind == MC_ENERGY_UNIFORM)          // Distribution.cpp:30-33
-                E0 = S.energy[0] + urand(rng) * (S.energy[1] - S.energy[0]);
-            else
-                E0 = S.energy[0];
-            A.x[i] = S.pos[0]; A.y[i] = S.pos[1]; A.z[i] = S.pos[2];
-            A.u[i] = ux; A.v[i] = uy; A.w[i] = uz;
-            A.E[i] = E0;
-            A.wgt[i] = 1.0;
-            A.rng[i] = rng;
-            A.hid[i] = hid;
-            A.pcell[i] = S.cell;
-            A.pmat[i] = A.cell[S.cell].material;
-            A.pprob[i] = p;
-            A.nflight[i] = 0;
-            A.tcount[i] = 0;
-            A.status[i] = ST_ALIVE;
-            if (kGrad) {
-                for (int k = 0; k < PR.param_count; ++k) {
-                    A.S[(size_t)k * A.cap + i] = 0.0;
-                    A.H[(size_t)k * A.cap + i] = 0.0;
-                }
-                A.Qw[i] = 0.0;
-                A.cs[i] = 0.0;
-                A.cc[i] = 0.0;
-                A.lpend[i] = 0.0;
-            }
-        } else if (need && st != ST_EMPTY) {
-            A.status[i] = ST_EMPTY;
+            ++fidx;
         }
-        if (keep || got) A.qA[act_run + arank] = i;
-        free_run += nfree;
-        act_run += nact;
     }
 }
 
@@ -586,77 +640,134 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
     return ST_DEAD;
 }
 
-// Distance to collision and to the nearest surface, the move and its
-// track-length scores; a flight ending on a surface crosses it here.
+// One flight of slot s: distance to collision and to the nearest surface,
+// the move and its track-length scores; a flight ending on a surface crosses
+// it here.  Returns the new status (ST_COLL: a collision is due).
+// kDefer: a flight ending on a surface stores the particle and the surface
+// and returns ST_SURF; the crossing is left to cross_deferred.
+template <bool kTables, bool kGrad, bool kDefer>
+__device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& crossed)
+{
+    const int c = A.pcell[s];
+    const mc_cell_desc C = A.cell[c];
+    Vec3 p = {A.x[s], A.y[s], A.z[s]};
+    const Vec3 u = {A.u[s], A.v[s], A.w[s]};
+    // surface_intersect, general.cpp:54-67
+    double dist = kMaxFloat;
+    int S = -1;
+    for (int k = 0; k < C.surf_count; ++k) {
+        const int sf = A.cell_surf[C.surf_begin + k];
+        const mc_surface_desc& Sd = A.surf[sf];
+        const double d = surface_distance(Sd.type, Sd.c, p, u);
+        if (d < dist) { dist = d; S = sf; }
+    }
+    // collision_distance, general.cpp:40-48
+    uint64_t rng = A.rng[s];
+    const double E = A.E[s];
+    double dcol, sigt = 0.0;
+    if (C.material >= 0) {
+        sigt = kTables ? A.sigt[s] : material_sigt(A, C.material, E);
+        dcol = -mc_log(urand(rng)) / sigt;
+    } else {
+        dcol = kMaxFloatLess;
+    }
+    const bool hit = dcol > dist;
+    const double l = hit ? dist : dcol;
+    const int p_id = A.pprob[s];
+    const DProb PR = A.prob[p_id];
+    // gradient runs: the flight's -l dSigma_t/dtheta is folded into S at the
+    // collision or surface crossing that ends it, where the per-nuclide
+    // cross sections at E are evaluated anyway
+    if (kGrad && C.material >= 0 && A.mat[C.material].param_count > 0) A.lpend[s] = l;
+    // move_particle, general.cpp:73-83
+    p.x += u.x * l;
+    p.y += u.y * l;
+    p.z += u.z * l;
+    const uint32_t nfl = A.nflight[s] + 1;
+    const double w = A.wgt[s];
+    for (int a = 0; a < C.tl_count; ++a)
+        score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, l, sigt, !hit);
+    uint8_t st;
+    crossed = false;
+    if (!hit) {
+        A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+        A.rng[s] = rng;
+        A.nflight[s] = nfl;
+        st = ST_COLL;
+    } else if (S >= 0) {
+        crossed = true;
+        if (kDefer) {
+            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+            A.rng[s] = rng;
+            A.nflight[s] = nfl;
+            A.psurf[s] = S;
+            st = ST_SURF;
+        } else {
+            st = surface_crossing<kGrad>(A, s, S, p, u, E, w, rng, nfl, c, p_id);
+        }
+    } else {                        // no surface ahead: lost particle
+        atomicAdd(&A.ctr->err_lost, 1u);
+        A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+        A.rng[s] = rng;
+        A.nflight[s] = nfl;
+        A.wgt[s] = 0.0;
+        finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, nfl);
+        st = ST_DEAD;
+    }
+    A.status[s] = st;
+    return st;
+}
+
+// The surface crossing of a flight that flight_one<..., true> deferred.
+template <bool kGrad>
+__device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
+{
+    const Vec3 p = {A.x[s], A.y[s], A.z[s]};
+    const Vec3 u = {A.u[s], A.v[s], A.w[s]};
+    A.status[s] = surface_crossing<kGrad>(A, s, A.psurf[s], p, u, A.E[s], A.wgt[s], A.rng[s], A.nflight[s],
+                                          A.pcell[s], A.pprob[s]);
+}
+
+// Flights of queue A over the block's range.  Flights that end on a surface
+// are queued in LDS (an LDS atomic gives each a place; the order does not
+// matter, every crossing is independent) and crossed by the whole block
+// once the queue fills or the range ends, so the rare crossing does not hold
+// a wave's other lanes and its registers stay out of the flight loop.
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
+    constexpr uint32_t kXq = 2048;      // >= the largest workgroup
     __shared__ uint32_t lds[32];
+    __shared__ uint32_t xq[kXq];
+    __shared__ uint32_t xn;
     const uint32_t n = A.ctr->nA;
     uint32_t lo, hi;
     block_range(n, lo, hi);
+    if (threadIdx.x == 0) xn = 0;
+    __syncthreads();
     uint32_t nsurf = 0, ncoll = 0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        const uint32_t s = A.qA[i];
-        const int c = A.pcell[s];
-        const mc_cell_desc C = A.cell[c];
-        Vec3 p = {A.x[s], A.y[s], A.z[s]};
-        const Vec3 u = {A.u[s], A.v[s], A.w[s]};
-        // surface_intersect, general.cpp:54-67
-        double dist = kMaxFloat;
-        int S = -1;
-        for (int k = 0; k < C.surf_count; ++k) {
-            const int sf = A.cell_surf[C.surf_begin + k];
-            const mc_surface_desc& Sd = A.surf[sf];
-            const double d = surface_distance(Sd.type, Sd.c, p, u);
-            if (d < dist) { dist = d; S = sf; }
+    for (uint32_t base = lo; base < hi; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        if (i < hi) {
+            const uint32_t s = A.qA[i];
+            bool crossed;
+            const uint8_t st = flight_one<kTables, kGrad, true>(A, s, crossed);
+            if (st == ST_COLL) {
+                ++ncoll;
+            } else if (st == ST_SURF) {
+                xq[atomicAdd(&xn, 1u)] = s;
+                ++nsurf;
+            }
         }
-        // collision_distance, general.cpp:40-48
-        uint64_t rng = A.rng[s];
-        const double E = A.E[s];
-        double dcol, sigt = 0.0;
-        if (C.material >= 0) {
-            sigt = kTables ? A.sigt[s] : material_sigt(A, C.material, E);
-            dcol = -mc_log(urand(rng)) / sigt;
-        } else {
-            dcol = kMaxFloatLess;
+        __syncthreads();
+        const uint32_t q = xn;          // the same in every thread between the barriers
+        __syncthreads();
+        if (q + blockDim.x > kXq || base + blockDim.x >= hi) {
+            for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
+            __syncthreads();
+            if (threadIdx.x == 0) xn = 0;
+            __syncthreads();
         }
-        const bool hit = dcol > dist;
-        const double l = hit ? dist : dcol;
-        const int p_id = A.pprob[s];
-        const DProb PR = A.prob[p_id];
-        // gradient runs: the flight's -l dSigma_t/dtheta is folded into S at the
-        // collision or surface crossing that ends it, where the per-nuclide
-        // cross sections at E are evaluated anyway
-        if (kGrad && C.material >= 0 && A.mat[C.material].param_count > 0) A.lpend[s] = l;
-        // move_particle, general.cpp:73-83
-        p.x += u.x * l;
-        p.y += u.y * l;
-        p.z += u.z * l;
-        const uint32_t nfl = A.nflight[s] + 1;
-        const double w = A.wgt[s];
-        for (int a = 0; a < C.tl_count; ++a)
-            score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, l, sigt, !hit);
-        uint8_t st;
-        if (!hit) {
-            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-            A.rng[s] = rng;
-            A.nflight[s] = nfl;
-            st = ST_COLL;
-            ++ncoll;
-        } else if (S >= 0) {
-            st = surface_crossing<kGrad>(A, s, S, p, u, E, w, rng, nfl, c, p_id);
-            ++nsurf;
-        } else {                        // no surface ahead: lost particle
-            atomicAdd(&A.ctr->err_lost, 1u);
-            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-            A.rng[s] = rng;
-            A.nflight[s] = nfl;
-            A.wgt[s] = 0.0;
-            finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, nfl);
-            st = ST_DEAD;
-        }
-        A.status[s] = st;
     }
     block_sum2(nsurf, ncoll, lds);
     if (threadIdx.x == 0) {
@@ -692,163 +803,236 @@ __global__ void __launch_bounds__(1024) k_event_scatter(KArgs A)
     }
 }
 
+// One collision of slot s (general.cpp:121-163): collision tallies, implicit
+// capture, nuclide selection, elastic scatter, weight roulette.  Returns the
+// new status.
+template <bool kTables, bool kGrad>
+__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
+{
+    const int c = A.pcell[s];
+    const mc_cell_desc C = A.cell[c];
+    const int p_id = A.pprob[s];
+    const DProb PR = A.prob[p_id];
+    double w = A.wgt[s];
+    uint64_t rng = A.rng[s];
+    double E = A.E[s];
+    bool alive = true;
+    Vec3 u = {A.u[s], A.v[s], A.w[s]};
+    if (C.material < 0) {                    // void: kill at collision
+        alive = false;
+        w = 0.0;
+    } else {
+        if (C.col_count > 0) {
+            const double lp = kGrad ? A.lpend[s] : 0.0;
+            for (int a = 0; a < C.col_count; ++a)
+                score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, lp, 0.0, false);
+        }
+        const int m = C.material;
+        const mc_material_desc M = A.mat[m];
+        const bool scaled = M.param_count != 0;
+        XsPoint P;
+        const int gm = kTables ? material_point_cached(A, m, s, E, P) : material_point(A, m, E, P);
+        // Sigma_s, Sigma_c (and Sigma_t when no lookup kernel ran) in one pass.
+        // The first kNucReg entries keep in registers what later steps reuse:
+        // the products sigma_s * N for the nuclide choice (the same terms
+        // Sigma_s sums) or, in gradient runs, the microscopic sigma_s, from
+        // which both the choice and dSigma_s/dtheta follow.
+        constexpr int kNucReg = 8;
+        double ps[kGrad ? 1 : kNucReg];
+        double rs[kGrad ? kNucReg : 1];
+        Xs3 X = {0.0, 0.0, 0.0};
+        // sigma_s and sigma_c of every nuclide from the material's record (one
+        // or two cache lines) when it has one, else from the nuclide tables
+        const DMatRec R = A.mrec[m];
+        const double* rsc = kTables && R.sc_off >= 0 ? rec_at(A.rec_sc, R.sc_off, R.sc_stride, P) : nullptr;
+        const int n2 = R.c_at;
+#pragma unroll
+        for (int k = 0; k < kNucReg; ++k) {
+            if (!kGrad) ps[k] = 0.0;
+            if (k < M.nuc_count) {
+                const int j = M.nuc_begin + k;
+                const double Nd = A.mat_dens[j];
+                const double xs_s = rsc ? xs_pair(rsc + 2 * k, P) : entry_xs(A, j, XS_S, E, P, gm);
+                const double xs_c = rsc ? xs_pair(rsc + n2 + 2 * k, P) : entry_xs(A, j, XS_C, E, P, gm);
+                const double prod = xs_s * (scaled ? A.mat_ns[j] : Nd);
+                if constexpr (kGrad) {
+                    rs[k] = xs_s;
+                } else {
+                    ps[k] = prod;
+                }
+                X.s += prod;
+                X.c += xs_c * (scaled ? A.mat_nc[j] : Nd);
+                if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
+            }
+        }
+        for (int k = kNucReg; k < M.nuc_count; ++k) {
+            const int j = M.nuc_begin + k;
+            const double Nd = A.mat_dens[j];
+            const double xs_s = rsc ? xs_pair(rsc + 2 * k, P) : entry_xs(A, j, XS_S, E, P, gm);
+            const double xs_c = rsc ? xs_pair(rsc + n2 + 2 * k, P) : entry_xs(A, j, XS_C, E, P, gm);
+            X.s += xs_s * (scaled ? A.mat_ns[j] : Nd);
+            X.c += xs_c * (scaled ? A.mat_nc[j] : Nd);
+            if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
+        }
+        // Sigma_t: the lookup kernel's value for this particle is the same sum
+        if (scaled) X.t = X.s + X.c;
+        else if (kTables) X.t = A.sigt[s];
+        // implicit capture, general.cpp:154-156 (Sigma_f = 0)
+        const double implicit = X.c + 0.0;
+        w = w * (X.t - implicit) / X.t;
+        // Material::nuclide_scatter, Material.cpp:106-115
+        const double ur = X.s * urand(rng);
+        double acc = 0.0;
+        int sel = -1;
+#pragma unroll
+        for (int k = 0; k < kNucReg; ++k) {
+            if (sel < 0 && k < M.nuc_count) {
+                if constexpr (kGrad) {
+                    const int j = M.nuc_begin + k;
+                    acc += rs[k] * (scaled ? A.mat_ns[j] : A.mat_dens[j]);
+                } else {
+                    acc += ps[k];
+                }
+                if (acc > ur) sel = k;
+            }
+        }
+        for (int k = kNucReg; sel < 0 && k < M.nuc_count; ++k) {
+            const int j = M.nuc_begin + k;
+            const double xs_s = rsc ? xs_pair(rsc + 2 * k, P) : entry_xs(A, j, XS_S, E, P, gm);
+            acc += xs_s * (scaled ? A.mat_ns[j] : A.mat_dens[j]);
+            if (acc > ur) sel = k;
+        }
+        if constexpr (kGrad) {
+            // Fold the flight that ended here (its 1/Sigma_t collision density
+            // cancels the implicit-capture weight factor's -dSigma_t/Sigma_t)
+            // and the nuclide selection, [scatter off sel]/theta, into S and H.
+            // dSigma_s/dtheta = sigma_s N and dSigma_c/dtheta = sigma_c N of the
+            // parameter's entry: sigma_s from the registers above, sigma_c
+            // read again (the record or table lines are in cache).
+            if (M.param_count > 0) {
+                const LrPending L = lr_pending(A, s);
+                for (int q = 0; q < M.param_count; ++q) {
+                    const int kg = A.mat_param[M.param_begin + q];
+                    const mc_param_desc D = A.param[kg];
+                    const int slot = D.nuc_slot, j = M.nuc_begin + slot;
+                    const double N = A.mat_dens[j];
+                    double ds = 0.0, dc = 0.0;
+                    if (D.reactions & MC_REACTION_SCATTER) {
+                        double xs_s = 0.0;
+                        if (slot < kNucReg) {
+#pragma unroll
+                            for (int t = 0; t < kNucReg; ++t)
+                                if (t == slot) xs_s = rs[t];
+                        } else {
+                            xs_s = rsc ? xs_pair(rsc + 2 * slot, P) : entry_xs(A, j, XS_S, E, P, gm);
+                        }
+                        ds = xs_s * N;
+                    }
+                    if (D.reactions & MC_REACTION_CAPTURE)
+                        dc = (rsc ? xs_pair(rsc + n2 + 2 * slot, P) : entry_xs(A, j, XS_C, E, P, gm)) * N;
+                    const double extra =
+                        (sel >= 0 && slot == sel && (D.reactions & MC_REACTION_SCATTER)) ? 1.0 / A.theta[kg] : 0.0;
+                    fold_one(A, s, kg - PR.param_begin, ds, dc, extra, L);
+                }
+                if (L.cs != 0.0 || L.cc != 0.0) {
+                    A.cs[s] = 0.0;
+                    A.cc[s] = 0.0;
+                }
+            }
+        }
+        if (sel >= 0) {
+            // elastic scatter, target at rest (T -> 0 limit of Reaction.cpp:27-118)
+            const DNuc N = A.nuc[A.mat_nuc[M.nuc_begin + sel]];
+            const double mu_cm = 2.0 * urand(rng) - 1.0;
+            double mu = mu_cm;
+            if (N.A < MC_MASS_INFINITE) {
+                const double Am = N.A;
+                const double q = Am * Am + 2.0 * Am * mu_cm + 1.0;
+                E = E * q / ((Am + 1.0) * (Am + 1.0));
+                mu = (1.0 + Am * mu_cm) / sqrt(q);
+            }
+            double ca, sa;
+            sincos_2pi(urand(rng), ca, sa);
+            u = scatter_direction(u, mu, ca, sa);
+        }
+    }
+    weight_roulette(w, alive, rng, PR);
+    A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
+    A.E[s] = E;
+    A.wgt[s] = w;
+    A.rng[s] = rng;
+    if (alive) {
+        A.status[s] = ST_ALIVE;
+        return ST_ALIVE;
+    } else {
+        const Vec3 p = {A.x[s], A.y[s], A.z[s]};
+        finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
+        A.status[s] = ST_DEAD;
+    }
+    return ST_DEAD;
+}
+
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_collision(KArgs A)
 {
     const uint32_t n = A.ctr->nC;
+    MC_GRID_LOOP(n) collision_one<kTables, kGrad>(A, A.qC[_i]);
+}
+
+// Drain of the last particles of a run.  Once no history is left to start and
+// the bank has thinned out, an event iteration costs its launch latencies for
+// few particles; here each thread instead carries its particles from event to
+// event (lookup, flight, surface crossing or collision) through the same
+// per-particle routines, so every history takes exactly the same steps.  At
+// most max_events flights per particle per launch; a particle still alive
+// after that stays in the bank for the next iteration.
+template <bool kTables, bool kGrad>
+__global__ void __launch_bounds__(1024) k_drain(KArgs A, uint32_t max_events)
+{
+    __shared__ uint32_t lds[32];
+    const uint32_t n = A.ctr->nA;
+    uint32_t nfl = 0, nsurf = 0, ncoll = 0;
     MC_GRID_LOOP(n)
     {
-        const uint32_t s = A.qC[_i];
-        const int c = A.pcell[s];
-        const mc_cell_desc C = A.cell[c];
-        const int p_id = A.pprob[s];
-        const DProb PR = A.prob[p_id];
-        double w = A.wgt[s];
-        uint64_t rng = A.rng[s];
-        double E = A.E[s];
-        bool alive = true;
-        Vec3 u = {A.u[s], A.v[s], A.w[s]};
-        if (C.material < 0) {                    // void: kill at collision
-            alive = false;
-            w = 0.0;
-        } else {
-            if (C.col_count > 0) {
-                const double lp = kGrad ? A.lpend[s] : 0.0;
-                for (int a = 0; a < C.col_count; ++a)
-                    score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, lp, 0.0, false);
-            }
-            const int m = C.material;
-            const mc_material_desc M = A.mat[m];
-            const bool scaled = M.param_count != 0;
-            XsPoint P;
-            const int gm = kTables ? material_point_cached(A, m, s, E, P) : material_point(A, m, E, P);
-            // Sigma_s, Sigma_c (and Sigma_t when no lookup kernel ran) in one pass.
-            // The first kNucReg entries keep in registers what later steps reuse:
-            // the products sigma_s * N for the nuclide choice (the same terms
-            // Sigma_s sums) or, in gradient runs, the microscopic sigma_s, from
-            // which both the choice and dSigma_s/dtheta follow.
-            constexpr int kNucReg = 8;
-            double ps[kGrad ? 1 : kNucReg];
-            double rs[kGrad ? kNucReg : 1];
-            Xs3 X = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int k = 0; k < kNucReg; ++k) {
-                if (!kGrad) ps[k] = 0.0;
-                if (k < M.nuc_count) {
-                    const int j = M.nuc_begin + k;
-                    const double Nd = A.mat_dens[j];
-                    const double xs_s = entry_xs(A, j, XS_S, E, P, gm);
-                    const double xs_c = entry_xs(A, j, XS_C, E, P, gm);
-                    const double prod = xs_s * (scaled ? A.mat_ns[j] : Nd);
-                    if constexpr (kGrad) {
-                        rs[k] = xs_s;
-                    } else {
-                        ps[k] = prod;
-                    }
-                    X.s += prod;
-                    X.c += xs_c * (scaled ? A.mat_nc[j] : Nd);
-                    if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
+        const uint32_t s = A.qA[_i];
+        for (uint32_t e = 0; e < max_events; ++e) {
+            if (kTables) {
+                const int m = A.pmat[s];
+                if (m >= 0) {
+                    int g = 0;
+                    A.sigt[s] = material_sigt(A, m, A.E[s], &g);
+                    A.gidx[s] = g;
                 }
             }
-            for (int k = kNucReg; k < M.nuc_count; ++k) {
-                const int j = M.nuc_begin + k;
-                const double Nd = A.mat_dens[j];
-                X.s += entry_xs(A, j, XS_S, E, P, gm) * (scaled ? A.mat_ns[j] : Nd);
-                X.c += entry_xs(A, j, XS_C, E, P, gm) * (scaled ? A.mat_nc[j] : Nd);
-                if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
+            bool crossed;
+            uint8_t st = flight_one<kTables, kGrad, false>(A, s, crossed);
+            ++nfl;
+            nsurf += crossed;
+            if (st == ST_COLL) {
+                ++ncoll;
+                st = collision_one<kTables, kGrad>(A, s);
             }
-            // Sigma_t: the lookup kernel's value for this particle is the same sum
-            if (scaled) X.t = X.s + X.c;
-            else if (kTables) X.t = A.sigt[s];
-            // implicit capture, general.cpp:154-156 (Sigma_f = 0)
-            const double implicit = X.c + 0.0;
-            w = w * (X.t - implicit) / X.t;
-            // Material::nuclide_scatter, Material.cpp:106-115
-            const double ur = X.s * urand(rng);
-            double acc = 0.0;
-            int sel = -1;
-#pragma unroll
-            for (int k = 0; k < kNucReg; ++k) {
-                if (sel < 0 && k < M.nuc_count) {
-                    if constexpr (kGrad) {
-                        const int j = M.nuc_begin + k;
-                        acc += rs[k] * (scaled ? A.mat_ns[j] : A.mat_dens[j]);
-                    } else {
-                        acc += ps[k];
-                    }
-                    if (acc > ur) sel = k;
-                }
-            }
-            for (int k = kNucReg; sel < 0 && k < M.nuc_count; ++k) {
-                const int j = M.nuc_begin + k;
-                acc += entry_xs(A, j, XS_S, E, P, gm) * (scaled ? A.mat_ns[j] : A.mat_dens[j]);
-                if (acc > ur) sel = k;
-            }
-            if constexpr (kGrad) {
-                // Fold the flight that ended here (its 1/Sigma_t collision density
-                // cancels the implicit-capture weight factor's -dSigma_t/Sigma_t)
-                // and the nuclide selection, [scatter off sel]/theta, into S and H.
-                // dSigma_s/dtheta = sigma_s N and dSigma_c/dtheta = sigma_c N of the
-                // parameter's entry: sigma_s from the registers above, sigma_c
-                // read again (the lines are in cache).
-                if (M.param_count > 0) {
-                    const LrPending L = lr_pending(A, s);
-                    for (int q = 0; q < M.param_count; ++q) {
-                        const int kg = A.mat_param[M.param_begin + q];
-                        const mc_param_desc D = A.param[kg];
-                        const int slot = D.nuc_slot, j = M.nuc_begin + slot;
-                        const double N = A.mat_dens[j];
-                        double ds = 0.0, dc = 0.0;
-                        if (D.reactions & MC_REACTION_SCATTER) {
-                            double xs_s = 0.0;
-                            if (slot < kNucReg) {
-#pragma unroll
-                                for (int t = 0; t < kNucReg; ++t)
-                                    if (t == slot) xs_s = rs[t];
-                            } else {
-                                xs_s = entry_xs(A, j, XS_S, E, P, gm);
-                            }
-                            ds = xs_s * N;
-                        }
-                        if (D.reactions & MC_REACTION_CAPTURE) dc = entry_xs(A, j, XS_C, E, P, gm) * N;
-                        const double extra =
-                            (sel >= 0 && slot == sel && (D.reactions & MC_REACTION_SCATTER)) ? 1.0 / A.theta[kg] : 0.0;
-                        fold_one(A, s, kg - PR.param_begin, ds, dc, extra, L);
-                    }
-                    if (L.cs != 0.0 || L.cc != 0.0) {
-                        A.cs[s] = 0.0;
-                        A.cc[s] = 0.0;
-                    }
-                }
-            }
-            if (sel >= 0) {
-                // elastic scatter, target at rest (T -> 0 limit of Reaction.cpp:27-118)
-                const DNuc N = A.nuc[A.mat_nuc[M.nuc_begin + sel]];
-                const double mu_cm = 2.0 * urand(rng) - 1.0;
-                double mu = mu_cm;
-                if (N.A < MC_MASS_INFINITE) {
-                    const double Am = N.A;
-                    const double q = Am * Am + 2.0 * Am * mu_cm + 1.0;
-                    E = E * q / ((Am + 1.0) * (Am + 1.0));
-                    mu = (1.0 + Am * mu_cm) / sqrt(q);
-                }
-                double ca, sa;
-                sincos_2pi(urand(rng), ca, sa);
-                u = scatter_direction(u, mu, ca, sa);
-            }
+            if (st != ST_ALIVE) break;
         }
-        weight_roulette(w, alive, rng, PR);
-        A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
-        A.E[s] = E;
-        A.wgt[s] = w;
-        A.rng[s] = rng;
-        if (alive) {
-            A.status[s] = ST_ALIVE;
-        } else {
-            const Vec3 p = {A.x[s], A.y[s], A.z[s]};
-            finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
-            A.status[s] = ST_DEAD;
-        }
+    }
+    // event counts: nfl and nsurf summed per block, then one atomic each
+    block_sum2(nfl, nsurf, lds);
+    uint32_t zero = 0;
+    block_sum2(ncoll, zero, lds);
+    if (threadIdx.x == 0) {
+        if (nfl) atomicAdd((unsigned long long*)&A.ctr->tA, (unsigned long long)nfl);
+        if (nsurf) atomicAdd((unsigned long long*)&A.ctr->tS, (unsigned long long)nsurf);
+        if (ncoll) atomicAdd((unsigned long long*)&A.ctr->tC, (unsigned long long)ncoll);
+    }
+}
+
+// After a drain: its events are counted, so the next roll-over adds none.
+__global__ void k_drain_close(Counters* c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->nA = 0;
+        c->nS = 0;
+        c->nC = 0;
     }
 }
 
@@ -939,6 +1123,10 @@ template __global__ void k_flight<false, false>(KArgs);
 template __global__ void k_flight<true, false>(KArgs);
 template __global__ void k_flight<false, true>(KArgs);
 template __global__ void k_flight<true, true>(KArgs);
+template __global__ void k_drain<false, false>(KArgs, uint32_t);
+template __global__ void k_drain<true, false>(KArgs, uint32_t);
+template __global__ void k_drain<false, true>(KArgs, uint32_t);
+template __global__ void k_drain<true, true>(KArgs, uint32_t);
 template __global__ void k_collision<false, false>(KArgs);
 template __global__ void k_collision<true, false>(KArgs);
 template __global__ void k_collision<false, true>(KArgs);
