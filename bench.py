@@ -80,6 +80,61 @@ def zscores(gm, gu, om, ou, min_rel=0.2):
     return (gm[ok] - om[ok]) / s[ok]
 
 
+def _pmc_pass(counter, xs, n, tmp, timeout=240):
+    """One rocprofv3 counter pass over the traffic probe.  Returns
+    ({kernel: summed counter}, largest copy-kernel value, probe json)."""
+    import csv
+    import subprocess
+    d = os.path.join(tmp, counter.lower())
+    cmd = ["rocprofv3", "--pmc", counter, "--kernel-trace", "--output-format", "csv", "-d", d, "-o", "run",
+           "--", sys.executable, os.path.join(ROOT, "mc-old_amd", "traffic_probe.py"), str(n), xs]
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout, env=env, cwd=ROOT)
+    probe = None
+    for line in r.stdout.splitlines():
+        if line.startswith("{"):
+            probe = json.loads(line)
+    path = None
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                path = os.path.join(root, f)
+    if probe is None or path is None:
+        raise RuntimeError(f"{counter} pass failed (rc {r.returncode}): {r.stderr[-400:]}")
+    sums, copy = {}, 0.0
+    for row in csv.DictReader(open(path)):
+        if row["Counter_Name"] != counter:
+            continue
+        name, v = row["Kernel_Name"], float(row["Counter_Value"])
+        if "copyBuffer" in name:
+            copy = max(copy, v)
+        if "mc::" in name:
+            k = name.split("mc::")[1].split("(")[0].split("<")[0]
+            sums[k] = sums.get(k, 0.0) + v
+    return sums, copy, probe
+
+
+def measure_traffic(xs, n=10_000_000):
+    """HBM bytes per particle event of every event-loop kernel from FETCH_SIZE
+    and WRITE_SIZE (KiB per dispatch), each scaled by what it reads for a
+    512 MiB copy.  Runs before this process touches the GPU."""
+    import shutil
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        f, fc, probe = _pmc_pass("FETCH_SIZE", xs, n, tmp)
+        w, wc, _ = _pmc_pass("WRITE_SIZE", xs, n, tmp)
+    cf = probe["copy_bytes"] / (fc * 1024.0)     # FETCH_SIZE reads half the copy on this part
+    cw = probe["copy_bytes"] / (wc * 1024.0)
+    per = {}
+    for k, items in probe["items"].items():
+        if items and k in f:
+            per[k] = {"fetch_B": f[k] * 1024.0 * cf / items, "write_B": w.get(k, 0.0) * 1024.0 * cw / items}
+    return {"copy_calibration": {"fetch_factor": cf, "write_factor": cw}, "histories": n,
+            "bytes_per_item": per}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +144,22 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
               "(histories shard with no communication; see the sharding design)", file=sys.stderr)
         sys.exit(2)
+
+    traffic = None
+    if not (args.quick or args.no_traffic):
+        # child processes under rocprofv3, started before this process loads the
+        # engine library or uses the GPU (the probe writes the nuclide files)
+        try:
+            traffic = measure_traffic(os.path.join(ROOT, "_bench_xs"))
+            log(f"traffic: {json.dumps(traffic)}")
+        except Exception as ex:      # the counters are a report, not a gate
+            log(f"traffic passes skipped: {ex}")
 
     mc = importlib.import_module("mc-old_amd")
     decks = importlib.import_module("mc-old_amd.decks")
@@ -167,10 +233,24 @@ def main():
                         "avg_us": 1e6 * s["seconds"] / s["launches"], "algo_GBs": gbs}
         dom = max((k for k in table if k in ALGO_BYTES), key=lambda k: table[k]["seconds"])
         d = table[dom]
+        epl = d["items"] / d["launches"]
+        tr = None
+        if traffic and dom in traffic["bytes_per_item"]:
+            t = traffic["bytes_per_item"][dom]
+            tr = (t["fetch_B"] + t["write_B"]) * epl
+            for k, row in table.items():
+                if k in traffic["bytes_per_item"]:
+                    row["hbm_B_per_item"] = traffic["bytes_per_item"][k]
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["algo_GBs"], "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                           "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": tr,
                            "bytes_per_event": ALGO_BYTES[dom], "avg_launch_us": d["avg_us"],
-                           "events_per_launch": d["items"] / d["launches"]}
+                           "events_per_launch": epl}
+        if traffic:
+            out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE / WRITE_SIZE over the traffic probe "
+                                                 f"({traffic['histories']} histories, plain launches), scaled by a "
+                                                 "512 MiB copy: fetch x{:.3f}, write x{:.3f}".format(
+                                                     traffic["copy_calibration"]["fetch_factor"],
+                                                     traffic["copy_calibration"]["write_factor"]))
         out["kernels"] = table
         log("kernels:", json.dumps(table))
 
@@ -208,10 +288,19 @@ def main():
         peng2.forward(mode=mc.MC_RUN_PLAIN)
         tp = time.perf_counter() - t
         rate_p = float(N[sub].sum()) / tp
+        # the same without the drain kernel: event iterations only, for reference
+        peng3 = mc.Engine([probs[i] for i in sub], use_graph=G, block=tune.block, grid_per_cu=tune.grid_per_cu,
+                          bank_capacity=int(tune.bank_capacity), drain_divisor=-1)
+        peng3.forward(n_histories=np.maximum(2, N[sub] // 1000), mode=mc.MC_RUN_PLAIN)
+        t = time.perf_counter()
+        peng3.forward(mode=mc.MC_RUN_PLAIN)
+        rate_p3 = float(N[sub].sum()) / (time.perf_counter() - t)
         out["batched_vs_plain"] = {"problems": 4096, "histories": int(N.sum()), "batched_seconds": tb,
                                    "batched_hist_per_s": rate_b, "plain_subset": int(sub.size),
                                    "plain_seconds": tp, "plain_hist_per_s": rate_p, "speedup": rate_b / rate_p,
-                                   "flights": int(br.flights), "autotuned": tuned}
+                                   "flights": int(br.flights), "autotuned": tuned,
+                                   "plain_event_iterations_only_hist_per_s": rate_p3,
+                                   "speedup_vs_event_iterations_only": rate_b / rate_p3}
         golden = os.path.join(ROOT, "tests", "golden", "batched_slabs_analytic.npz")
         if os.path.exists(golden):
             g = np.load(golden)
@@ -228,10 +317,9 @@ def main():
         # ------------------------------------------------ backward cost
         p3 = mc.Problem.from_string(decks.shield_deck(H, params="24"), xs)
         e3 = mc.Engine(p3, use_graph=G)
-        n3 = max(2, H // 10)
+        n3 = H                                    # the differentiable-shield case runs 1e8 histories
         e3.forward(n_histories=n3 // 10)
-        go = np.zeros(e3.n_bins)
-        go[0] = 1.0
+        go = np.ones(e3.n_bins)          # VJP weights: the transmitted fraction and every leakage bin
         e3.backward(go, n_histories=n3 // 10)
         t = time.perf_counter()
         f3 = e3.forward(n_histories=n3)
@@ -240,7 +328,7 @@ def main():
         b3 = e3.backward(go, n_histories=n3)
         tbw = time.perf_counter() - t
         out["bwd_fwd"] = {"histories": n3, "forward_s": tf, "backward_s": tbw, "ratio": tbw / tf,
-                          "params": int(e3.n_params),
+                          "params": int(e3.n_params), "grad_out": "ones over all 201 tally bins",
                           "dT_dtheta_max_rel_sigma": float(np.max(b3.grad_sigma / np.maximum(np.abs(b3.grad), 1e-300)))}
         log(f"bwd/fwd: {tbw:.3f}/{tf:.3f} = {tbw / tf:.2f}")
 

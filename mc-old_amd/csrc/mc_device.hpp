@@ -86,6 +86,14 @@ struct DMatLook {
     int32_t rec_stride, nuc_count, nuc_begin, scaled;
 };
 
+// One parameter of a material as the collision's likelihood-ratio fold reads
+// it, packed per run: problem-local index, nuclide slot, reactions, the
+// entry's density N and 1/theta.
+struct DParFold {
+    int32_t kl, slot, reactions, pad;
+    double N, inv_theta;
+};
+
 // Everything a kernel needs, passed by value.
 struct KArgs {
     // problem data
@@ -110,6 +118,7 @@ struct KArgs {
     const double* rec_sc;
     const mc_param_desc* param;   // material/nuc_slot rebased
     const double* theta;
+    const DParFold* mpar;         // per mat_param entry (gradient runs)
     const DNuc* nuc;
     const DGrid* grid;
     const double* grid_E;

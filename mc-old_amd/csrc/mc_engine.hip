@@ -114,7 +114,7 @@ struct Engine::Impl {
 
     // device data
     DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
-        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_t, d_rec_sc, d_param, d_theta, d_nuc, d_grid,
+        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
         d_run_hb;
@@ -547,6 +547,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.rec_sc = d_rec_sc.as<double>();
     A.param = d_param.as<mc_param_desc>();
     A.theta = d_theta.as<double>();
+    A.mpar = d_mpar.as<DParFold>();
     A.nuc = d_nuc.as<DNuc>();
     A.grid = d_grid.as<DGrid>();
     A.grid_E = d_grid_E.as<double>();
@@ -794,6 +795,25 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     upload(d_theta, th, stream);
     upload(d_mat_ns, ns, stream);
     upload(d_mat_nc, nc, stream);
+    // the collision fold's packed parameter table (1/theta as the kernels would
+    // divide it: IEEE division on both sides)
+    std::vector<int> owner(std::max(1, n_params), 0);
+    for (int p = 0; p < np; ++p)
+        for (int k = prob[p].param_begin; k < prob[p].param_begin + prob[p].param_count; ++k) owner[k] = p;
+    std::vector<DParFold> mpar(std::max<size_t>(1, mat_param.size()));
+    for (size_t m = 0; m < mat.size(); ++m)
+        for (int q = 0; q < mat[m].param_count; ++q) {
+            const int kg = mat_param[mat[m].param_begin + q];
+            const mc_param_desc& D = param[kg];
+            DParFold& F = mpar[mat[m].param_begin + q];
+            F.kl = kg - prob[owner[kg]].param_begin;
+            F.slot = D.nuc_slot;
+            F.reactions = D.reactions;
+            F.pad = 0;
+            F.N = mat_dens[mat[m].nuc_begin + D.nuc_slot];
+            F.inv_theta = 1.0 / th[kg];
+        }
+    upload(d_mpar, mpar, stream);
     HIP_CHECK(hipMemcpyAsync(d_run_hb.p, hb.data(), sizeof(uint64_t) * (np + 1), hipMemcpyHostToDevice, stream));
     if (grad) {
         std::vector<double> g(gout, gout + n_bins);

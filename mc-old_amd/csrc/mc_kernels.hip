@@ -324,7 +324,9 @@ __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& P
             T.score = tot;
         }
     }
-    if (kGrad) {
+    // gradient: a history that touched no bin made no score with v != 0, so
+    // Q, H and the pending coefficients are all zero and it adds nothing
+    if (kGrad && nt != 0) {
         // scores still pending (surface scores of the cell the particle ends in)
         double* Srow = A.S + s;
         double* Hrow = A.H + s;
@@ -746,10 +748,12 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     if (threadIdx.x == 0) xn = 0;
     __syncthreads();
     uint32_t nsurf = 0, ncoll = 0;
+    uint32_t s_next = lo + threadIdx.x < hi ? A.qA[lo + threadIdx.x] : 0u;   // read one item ahead
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
+        const uint32_t s = s_next;
+        if (i + blockDim.x < hi) s_next = A.qA[i + blockDim.x];
         if (i < hi) {
-            const uint32_t s = A.qA[i];
             bool crossed;
             const uint8_t st = flight_one<kTables, kGrad, true>(A, s, crossed);
             if (st == ST_COLL) {
@@ -912,12 +916,11 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
             if (M.param_count > 0) {
                 const LrPending L = lr_pending(A, s);
                 for (int q = 0; q < M.param_count; ++q) {
-                    const int kg = A.mat_param[M.param_begin + q];
-                    const mc_param_desc D = A.param[kg];
-                    const int slot = D.nuc_slot, j = M.nuc_begin + slot;
-                    const double N = A.mat_dens[j];
+                    const DParFold F = A.mpar[M.param_begin + q];
+                    const int slot = F.slot, j = M.nuc_begin + slot;
+                    const double N = F.N;
                     double ds = 0.0, dc = 0.0;
-                    if (D.reactions & MC_REACTION_SCATTER) {
+                    if (F.reactions & MC_REACTION_SCATTER) {
                         double xs_s = 0.0;
                         if (slot < kNucReg) {
 #pragma unroll
@@ -928,11 +931,11 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
                         }
                         ds = xs_s * N;
                     }
-                    if (D.reactions & MC_REACTION_CAPTURE)
+                    if (F.reactions & MC_REACTION_CAPTURE)
                         dc = (rsc ? xs_pair(rsc + n2 + 2 * slot, P) : entry_xs(A, j, XS_C, E, P, gm)) * N;
                     const double extra =
-                        (sel >= 0 && slot == sel && (D.reactions & MC_REACTION_SCATTER)) ? 1.0 / A.theta[kg] : 0.0;
-                    fold_one(A, s, kg - PR.param_begin, ds, dc, extra, L);
+                        (sel >= 0 && slot == sel && (F.reactions & MC_REACTION_SCATTER)) ? F.inv_theta : 0.0;
+                    fold_one(A, s, F.kl, ds, dc, extra, L);
                 }
                 if (L.cs != 0.0 || L.cc != 0.0) {
                     A.cs[s] = 0.0;
@@ -975,8 +978,16 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_collision(KArgs A)
 {
+    // the next queue entry is read one item ahead, off the item's load chain
     const uint32_t n = A.ctr->nC;
-    MC_GRID_LOOP(n) collision_one<kTables, kGrad>(A, A.qC[_i]);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t s_next = i < n ? A.qC[i] : 0u;
+    for (; i < n; i += stride) {
+        const uint32_t s = s_next;
+        if (i + stride < n) s_next = A.qC[i + stride];
+        collision_one<kTables, kGrad>(A, s);
+    }
 }
 
 // Drain of the last particles of a run.  Once no history is left to start and

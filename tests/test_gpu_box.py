@@ -12,10 +12,11 @@ pytestmark = pytest.mark.gpu
 THIN = dict(box_x=10.0, detector=(40.0, 0.0, 0.0, 20.0))
 
 
-def test_box_histories_bitexact():
+@pytest.mark.parametrize("drain", [1, -1])
+def test_box_histories_bitexact(drain):
     n = 6000
     p = mc.Problem.from_string(decks.box_deck(n, **THIN), shield_xs_dir())
-    g = mc.Engine(p, bank_capacity=1 << 12).forward(trace=n)
+    g = mc.Engine(p, bank_capacity=1 << 12, drain_divisor=drain).forward(trace=n)
     o = oracle_run(p, n, trace=n)
     for f in ("rng", "flights", "x", "y", "z", "E", "score"):
         assert (g.trace[f] == o.trace[f]).all(), f

@@ -35,10 +35,11 @@ def test_reaction_score_backward_matches_oracle():
     np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12)
 
 
-def test_shield_backward_matches_oracle():
-    n = 3000
+@pytest.mark.parametrize("drain", [1, -1])
+def test_shield_backward_matches_oracle(drain):
+    n = 6000
     p = mc.Problem.from_string(decks.shield_deck(n, params="24"), shield_xs_dir())
-    eng = mc.Engine(p, bank_capacity=1 << 12)
+    eng = mc.Engine(p, bank_capacity=1 << 12, drain_divisor=drain)
     rng = np.random.default_rng(2)
     go = rng.normal(size=p.n_bins)
     theta = rng.uniform(0.9, 1.1, p.n_params)

@@ -41,10 +41,12 @@ def test_device_math_bitexact():
             assert d[i, 1] == c.value and d[i, 2] == s.value, xi
 
 
-def test_slab_histories_bitexact():
+# drain_divisor 1: the run's tail goes to the drain kernel; -1: event iterations only
+@pytest.mark.parametrize("drain", [1, -1])
+def test_slab_histories_bitexact(drain):
     n = 20000
     p = mc.Problem.from_string(decks.slab_deck(4.0, 0.5, 100, n))
-    g = mc.Engine(p, bank_capacity=1 << 13).forward(trace=n)
+    g = mc.Engine(p, bank_capacity=1 << 13, drain_divisor=drain).forward(trace=n)
     o = oracle_run(p, n, trace=n)
     assert_traces_equal(g.trace, o.trace, "slab")
     assert g.flights == o.flights
@@ -76,10 +78,11 @@ def test_slab_full_size_vs_analytic():
     assert abs(g.mean[0] + g.mean[1] + 0.5 * g.mean[2:].sum() - 1.0) < 3e-3
 
 
-def test_shield_histories_bitexact():
-    n = 3000
+@pytest.mark.parametrize("drain", [1, -1])
+def test_shield_histories_bitexact(drain):
+    n = 6000
     p = mc.Problem.from_string(decks.shield_deck(n), shield_xs_dir())
-    g = mc.Engine(p, bank_capacity=1 << 12).forward(trace=n)
+    g = mc.Engine(p, bank_capacity=1 << 12, drain_divisor=drain).forward(trace=n)
     o = oracle_run(p, n, trace=n)
     assert_traces_equal(g.trace, o.trace, "shield")
     assert g.flights == o.flights
