@@ -265,7 +265,7 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
                         c.bank_capacity = B;
                         c.iters_per_graph = 16;
                         c.drain_divisor = dd;
-                        c.use_graph = 1;
+                        c.use_graph = base.use_graph;   // tune the launch mode the engine uses
                         c.profile = 0;
                         E.set_cfg(c);
                         mc_tally_out t{};
