@@ -135,6 +135,17 @@ def measure_traffic(xs, n=10_000_000):
             "bytes_per_item": per}
 
 
+def xs_lookup_bench(mc, decks, xs, n_xs=1 << 26):
+    """k_xs_lookup_array over 2^26 (energy, material) pairs of the shield's
+    materials: 20 algorithmic bytes per lookup (energy, material in; Sigma_t out)."""
+    eng = mc.Engine(mc.Problem.from_string(decks.shield_deck(1000), xs))
+    sec, _ = eng.xs_bench(n_xs, iters=20)
+    gbs = 20 * n_xs / sec / 1e9
+    log(f"xs lookup: {n_xs / sec:.4g} lookups/s, {gbs:.1f} GB/s ({gbs / HBM_PEAK_GBS:.1%})")
+    return {"lookups": n_xs, "seconds_per_launch": sec, "lookups_per_s": n_xs / sec,
+            "algo_bytes_per_lookup": 20, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,12 +156,19 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
+    ap.add_argument("--xs-only", action="store_true", help="only the standalone cross-section lookup")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
               "(histories shard with no communication; see the sharding design)", file=sys.stderr)
         sys.exit(2)
 
+    if args.xs_only:
+        mc = importlib.import_module("mc-old_amd")
+        decks = importlib.import_module("mc-old_amd.decks")
+        xs = shield_xs_dir()
+        print(json.dumps(xs_lookup_bench(mc, decks, xs)), flush=True)
+        return
     traffic = None
     if not (args.quick or args.no_traffic):
         # child processes under rocprofv3, started before this process loads the
@@ -255,12 +273,7 @@ def main():
         log("kernels:", json.dumps(table))
 
         # ------------------------------------------ standalone XS lookup
-        n_xs = 1 << 26
-        sec, _ = eng.xs_bench(n_xs, iters=20)
-        gbs = 20 * n_xs / sec / 1e9
-        out["xs_lookup"] = {"lookups": n_xs, "seconds_per_launch": sec, "lookups_per_s": n_xs / sec,
-                            "algo_bytes_per_lookup": 20, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
-        log(f"xs lookup: {n_xs / sec:.4g} lookups/s, {gbs:.1f} GB/s ({gbs / HBM_PEAK_GBS:.1%})")
+        out["xs_lookup"] = xs_lookup_bench(mc, decks, xs)
 
         # -------------------------------------- batched vs plain sweep
         L, c, N = decks.batched_slabs(4096)
