@@ -84,7 +84,16 @@ struct DMatLook {
     int64_t rec_off;
     int32_t e_off, n, b_off, nb;
     int32_t rec_stride, nuc_count, nuc_begin, scaled;
+    int64_t te_off;               // rec_te record 0 (unscaled materials), -1: none
+    int32_t te_stride, grid;
 };
+
+// rec_te: record i = (E_i, E_{i+1}, then the (sigma_t[i], sigma_t[i+1]) pair of
+// each nuclide) for the lookup kernels' LDS path, whose bucket table sits in
+// LDS: bucket -> one record line (a second one when E lies above the bucket's
+// grid point).
+constexpr int kTeVals = 2 + 2 * kFastNuc;
+constexpr size_t kLdsBucketBytes = 96 * 1024;   // largest bucket table staged in LDS
 
 // One parameter of a material as the collision's likelihood-ratio fold reads
 // it, packed per run: problem-local index, nuclide slot, reactions, the
@@ -114,6 +123,9 @@ struct KArgs {
     const double* me_cst;         // per material entry: constant (t, s, c)
     const DMatRec* mrec;          // per material: interleaved cross-section records
     const DMatLook* mlook;        // per material: fast Sigma_t path
+    const double* rec_te;
+    const uint16_t* bucket16;     // the LDS grid's bucket table as lb + 1
+    int32_t lds_grid, lds_nb;     // grid staged in LDS by the lookup kernels (-1: none)
     const double* rec_t;
     const double* rec_sc;
     const mc_param_desc* param;   // material/nuc_slot rebased
@@ -418,6 +430,74 @@ __device__ __forceinline__ double material_sigt(const KArgs& A, int m, double E,
         c += entry_xs(A, j, XS_C, E, P, g) * A.mat_nc[j];
     }
     return s + c;
+}
+
+// Sigma_t of an unscaled material on the LDS grid: the bucket from LDS, then
+// the record of the interval's lower point, which carries the grid energies
+// the choice and the interpolation need.  Same interval, same values, same
+// arithmetic as material_sigt.
+__device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLook& L, const uint16_t* sb, double E,
+                                                   int& idx)
+{
+    const int64_t key = (int64_t)(double_to_bits(E) >> kBucketShift) - L.key_base;
+    const double* base = A.rec_te + L.te_off;
+    const int64_t st = L.te_stride;
+    double v[kTeVals];
+    int i;
+    if (key < 0 || key >= L.nb) {
+        i = key < 0 ? -1 : L.n - 1;
+        const double* r = base + (i > 0 ? i : 0) * st;
+#pragma unroll
+        for (int q = 0; q < kTeVals; ++q) v[q] = r[q];
+    } else {
+        const int lb = (int)sb[key] - 1, rb = (int)sb[key + 1] - 1;
+        const double* r = base + (lb > 0 ? lb : 0) * st;
+#pragma unroll
+        for (int q = 0; q < kTeVals; ++q) v[q] = r[q];
+        // the bucket's one grid point (E_{lb+1}) lies below E: the next interval
+        const bool up = rb > lb && (lb >= 0 ? v[1] : v[0]) < E;
+        i = up ? lb + 1 : lb;
+        if (up && lb >= 0) {
+            const double* r2 = base + (int64_t)(lb + 1) * st;
+#pragma unroll
+            for (int q = 0; q < kTeVals; ++q) v[q] = r2[q];
+        }
+    }
+    XsPoint P;
+    P.idx = i;
+    P.n = L.n;
+    if (i >= 0 && i < L.n - 1) {
+        P.a = (E - v[1]) / (v[0] - v[1]);
+        P.b = (E - v[0]) / (v[1] - v[0]);
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kFastNuc; ++k)
+        if (k < L.nuc_count) t += xs_pair(v + 2 + 2 * k, P) * A.mat_dens[L.nuc_begin + k];
+    idx = i;
+    return t;
+}
+
+// The lookup kernels' Sigma_t: the LDS path when the material has one, else
+// material_sigt.
+__device__ __forceinline__ double lookup_sigt(const KArgs& A, const uint16_t* sb, int m, double E, int& idx)
+{
+    if (sb) {
+        const DMatLook L = A.mlook[m];
+        if (L.te_off >= 0 && L.grid == A.lds_grid) return material_sigt_lds(A, L, sb, E, idx);
+    }
+    return material_sigt(A, m, E, &idx);
+}
+
+// Stages the LDS grid's bucket table (lds_nb + 1 entries) in dynamic LDS;
+// returns nullptr when there is none.  Every thread of the block must call it.
+__device__ __forceinline__ const uint16_t* stage_buckets(const KArgs& A, uint16_t* sb)
+{
+    if (A.lds_grid < 0) return nullptr;
+    const int n = A.lds_nb + 1;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) sb[k] = A.bucket16[k];
+    __syncthreads();
+    return sb;
 }
 
 __device__ __forceinline__ Xs3 material_xs3(const KArgs& A, int m, double E)

@@ -23,7 +23,7 @@ namespace mc {
 __global__ void k_refill_count(KArgs A);
 template <bool kGrad> __global__ void k_refill_scatter(KArgs A);
 __global__ void k_event_scatter(KArgs A);
-__global__ void k_xs_lookup(KArgs A);
+__global__ void k_xs_lookup(KArgs A, int use_lds);
 template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
@@ -92,6 +92,9 @@ struct Engine::Impl {
     std::vector<double> me_cst;
     std::vector<DMatRec> mrec;
     std::vector<DMatLook> mlook;
+    std::vector<double> rec_te;
+    std::vector<uint16_t> bucket16;
+    int lds_grid = -1, lds_nb = 0;
     std::vector<double> rec_t, rec_sc;
     std::vector<double> mat_dens;
     std::vector<mc_param_desc> param;
@@ -114,7 +117,7 @@ struct Engine::Impl {
 
     // device data
     DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
-        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nuc, d_grid,
+        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
         d_run_hb;
@@ -159,6 +162,11 @@ struct Engine::Impl {
     void run(const mc_run_cfg* run, bool grad, const double* gout, mc_tally_out* out, double* grad_mean,
              double* grad_sigma);
     dim3 grid_dim() const { return dim3((unsigned)std::max(1, n_cu * cfg.grid_per_cu)); }
+    // Launch shape of the standalone lookup kernel: with a bucket table in LDS
+    // it takes most of a CU's LDS, so one workgroup of 1024 per CU.
+    size_t lookup_lds() const { return lds_grid >= 0 ? ((size_t)(lds_nb + 1) * 2 + 15) / 16 * 16 : 0; }
+    dim3 lookup_grid() const { return lds_grid >= 0 ? dim3((unsigned)n_cu) : grid_dim(); }
+    dim3 lookup_block() const { return lds_grid >= 0 ? dim3(1024) : dim3((unsigned)cfg.block); }
 };
 
 // ---------------------------------------------------------------- building
@@ -383,6 +391,8 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         // fast Sigma_t path: records, at most kFastNuc nuclides, one point per bucket
         DMatLook L{};
         L.rec_off = -1;
+        L.te_off = -1;
+        L.grid = mat_grid[m];
         if (R.t_off >= 0 && n <= kFastNuc) {
             const DGrid& G = grid[mat_grid[m]];
             int crowd = 0;
@@ -398,11 +408,42 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
                 L.nb = G.nb;
                 L.nuc_count = n;
                 L.nuc_begin = mat[m].nuc_begin;
+                if (!L.scaled) {
+                    // records with their grid energies for the LDS path
+                    L.te_stride = (2 + 2 * n + 7) / 8 * 8;
+                    L.te_off = (int64_t)rec_te.size();
+                    rec_te.resize(rec_te.size() + (size_t)G.n * L.te_stride, 0.0);
+                    for (int i = 0; i < G.n; ++i) {
+                        const int i2 = i + 1 < G.n ? i + 1 : i;
+                        double* r = rec_te.data() + L.te_off + (int64_t)i * L.te_stride;
+                        r[0] = grid_E[G.e_off + i];
+                        r[1] = grid_E[G.e_off + i2];
+                        for (int k = 0; k < n; ++k) {
+                            const int tb = me_tab[mat[m].nuc_begin + k];
+                            r[2 + 2 * k] = tab_t[tb + i];
+                            r[3 + 2 * k] = tab_t[tb + i2];
+                        }
+                    }
+                }
             }
         }
         mlook.push_back(L);
     }
     if (rec_t.empty()) { rec_t.push_back(0.0); rec_sc.push_back(0.0); }
+    rec_te.resize(rec_te.size() + kTeVals, 0.0);   // a record read past the last one stays in bounds
+    // the grid whose bucket table the lookup kernels stage in LDS: the one of
+    // the first material with an LDS path, if its table fits and indices fit u16
+    for (const DMatLook& L : mlook) {
+        if (L.te_off < 0) continue;
+        const DGrid& G = grid[L.grid];
+        if (G.n < 65535 && (size_t)(G.nb + 1) * 2 <= kLdsBucketBytes) {
+            lds_grid = L.grid;
+            lds_nb = G.nb;
+            for (int k = 0; k <= G.nb; ++k) bucket16.push_back((uint16_t)(bucket[G.b_off + k] + 1));
+        }
+        break;
+    }
+    if (bucket16.empty()) bucket16.push_back(0);
     // per surface: cells that list it, in index order
     std::vector<std::vector<int>> lists(surf.size());
     for (size_t c = 0; c < cell.size(); ++c)
@@ -440,6 +481,8 @@ void Engine::Impl::upload_static()
     upload(d_me_cst, me_cst, stream);
     upload(d_mrec, mrec, stream);
     upload(d_mlook, mlook, stream);
+    upload(d_rec_te, rec_te, stream);
+    upload(d_bucket16, bucket16, stream);
     upload(d_rec_t, rec_t, stream);
     upload(d_rec_sc, rec_sc, stream);
     upload(d_param, param, stream);
@@ -543,6 +586,10 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.me_cst = d_me_cst.as<double>();
     A.mrec = d_mrec.as<DMatRec>();
     A.mlook = d_mlook.as<DMatLook>();
+    A.rec_te = d_rec_te.as<double>();
+    A.bucket16 = d_bucket16.as<uint16_t>();
+    A.lds_grid = lds_grid;
+    A.lds_nb = lds_nb;
     A.rec_t = d_rec_t.as<double>();
     A.rec_sc = d_rec_sc.as<double>();
     A.param = d_param.as<mc_param_desc>();
@@ -627,7 +674,7 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     else hipLaunchKernelGGL(k_refill_scatter<false>, g, b, 0, st, A);
     mark(1, 1);
     mark(2, 0);
-    if (has_tables) hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A);
+    if (has_tables) hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A, 0);
     mark(2, 1);
     mark(3, 0);
     if (has_tables) {
@@ -944,6 +991,11 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     HIP_CHECK(hipEventCreate(&d->ev1));
     d->build(ps);
     d->upload_static();
+    if (d->lookup_lds() > 0) {
+        // the standalone lookup stages up to kLdsBucketBytes of bucket table in LDS
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_xs_lookup_array, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)d->lookup_lds()));
+    }
 }
 
 Engine::~Engine() = default;
@@ -992,7 +1044,8 @@ void Engine::xs_lookup(const double* E, const int32_t* mat, double* out, uint64_
     HIP_CHECK(hipMemcpyAsync(bM.p, mat, 4 * n, hipMemcpyHostToDevice, d->stream));
     // problem 0's materials start at index 0
     const KArgs A = d->args(false, nullptr, 0);
-    hipLaunchKernelGGL(k_xs_lookup_array, d->grid_dim(), dim3(d->cfg.block), 0, d->stream, A, bE.as<double>(),
+    hipLaunchKernelGGL(k_xs_lookup_array, d->lookup_grid(), d->lookup_block(), d->lookup_lds(), d->stream, A,
+                       bE.as<double>(),
                        bM.as<int32_t>(), bO.as<double>(), n);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(out, bO.p, 8 * n, hipMemcpyDeviceToHost, d->stream));
@@ -1014,12 +1067,13 @@ void Engine::xs_bench(uint64_t n, int iters, uint64_t seed, double* sec, double*
     hipLaunchKernelGGL(k_xs_inputs, dim3(2048), dim3(256), 0, d->stream, bE.as<double>(), bM.as<int32_t>(), n,
                        lnmin, lnspan, bmats.as<int32_t>(), (int32_t)mats.size(), seed);
     const KArgs A = d->args(false, nullptr, 0);
-    const dim3 g = d->grid_dim(), b(d->cfg.block);
-    hipLaunchKernelGGL(k_xs_lookup_array, g, b, 0, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
+    const dim3 g = d->lookup_grid(), b = d->lookup_block();
+    const size_t lds = d->lookup_lds();
+    hipLaunchKernelGGL(k_xs_lookup_array, g, b, lds, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
                        bO.as<double>(), n);   // warm-up
     HIP_CHECK(hipEventRecord(d->ev0, d->stream));
     for (int k = 0; k < iters; ++k)
-        hipLaunchKernelGGL(k_xs_lookup_array, g, b, 0, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
+        hipLaunchKernelGGL(k_xs_lookup_array, g, b, lds, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
                            bO.as<double>(), n);
     HIP_CHECK(hipEventRecord(d->ev1, d->stream));
     HIP_CHECK(hipEventSynchronize(d->ev1));

@@ -534,8 +534,15 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
     }
 }
 
-__global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A)
+// Dynamic LDS of the lookup kernels: the bucket table of the LDS grid.
+extern __shared__ uint16_t mc_lds_buckets[];
+
+// use_lds: stage the bucket table in LDS (worth it when a launch has many
+// lookups spread over the whole grid; in the event loop the energies are
+// concentrated and the table lines are in cache, so the loop passes 0).
+__global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A, int use_lds)
 {
+    const uint16_t* sb = use_lds ? stage_buckets(A, mc_lds_buckets) : nullptr;
     // two independent lookups per thread per step keep more gathers in flight
     const uint32_t n = A.ctr->nA;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -546,8 +553,8 @@ __global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A)
         const int m = A.pmat[s], m2 = A.pmat[s2];
         const double E = A.E[s], E2 = A.E[s2];
         int g = 0, g2 = 0;
-        const double r = m >= 0 ? material_sigt(A, m, E, &g) : 0.0;
-        const double r2 = m2 >= 0 ? material_sigt(A, m2, E2, &g2) : 0.0;
+        const double r = m >= 0 ? lookup_sigt(A, sb, m, E, g) : 0.0;
+        const double r2 = m2 >= 0 ? lookup_sigt(A, sb, m2, E2, g2) : 0.0;
         // the interval is kept for the collision and surface kernels (same E)
         A.sigt[s] = r;
         A.gidx[s] = g;
@@ -1052,14 +1059,16 @@ __global__ void __launch_bounds__(1024) k_xs_lookup_array(KArgs A, const double*
                                                          const int32_t* __restrict__ mat,
                                                          double* __restrict__ out, uint64_t n)
 {
+    const uint16_t* sb = stage_buckets(A, mc_lds_buckets);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
         const uint64_t i2 = i + stride;
         const bool two = i2 < n;
         const int m = mat[i], m2 = two ? mat[i2] : -1;
         const double e = E[i], e2 = two ? E[i2] : 1.0;
-        const double r = m >= 0 ? material_sigt(A, m, e) : 0.0;
-        const double r2 = m2 >= 0 ? material_sigt(A, m2, e2) : 0.0;
+        int g = 0, g2 = 0;
+        const double r = m >= 0 ? lookup_sigt(A, sb, m, e, g) : 0.0;
+        const double r2 = m2 >= 0 ? lookup_sigt(A, sb, m2, e2, g2) : 0.0;
         out[i] = r;
         if (two) out[i2] = r2;
     }
