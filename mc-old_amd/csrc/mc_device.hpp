@@ -88,11 +88,11 @@ struct DMatLook {
     int32_t te_stride, grid;
 };
 
-// rec_te: record i = (E_i, E_{i+1}, then the (sigma_t[i], sigma_t[i+1]) pair of
-// each nuclide) for the lookup kernels' LDS path, whose bucket table sits in
-// LDS: bucket -> one record line (a second one when E lies above the bucket's
-// grid point).
-constexpr int kTeVals = 2 + 2 * kFastNuc;
+// rec_te: record i = (E_i, sigma_t[i] of each nuclide in material order), 32 or
+// 64 B, for the standalone lookup's LDS path, whose bucket table sits in LDS:
+// bucket -> the two records of the interval (adjacent, one or two lines; a
+// third when E lies above the bucket's grid point).
+constexpr int kTeVals = 1 + kFastNuc;
 constexpr size_t kLdsBucketBytes = 96 * 1024;   // largest bucket table staged in LDS
 
 // One parameter of a material as the collision's likelihood-ratio fold reads
@@ -442,38 +442,41 @@ __device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLo
     const int64_t key = (int64_t)(double_to_bits(E) >> kBucketShift) - L.key_base;
     const double* base = A.rec_te + L.te_off;
     const int64_t st = L.te_stride;
-    double v[kTeVals];
+    const int last = L.n - 1;
+    double a[kTeVals], b[kTeVals];       // records i and i + 1 of the interval
     int i;
     if (key < 0 || key >= L.nb) {
-        i = key < 0 ? -1 : L.n - 1;
+        i = key < 0 ? -1 : last;
         const double* r = base + (i > 0 ? i : 0) * st;
 #pragma unroll
-        for (int q = 0; q < kTeVals; ++q) v[q] = r[q];
+        for (int q = 0; q < kTeVals; ++q) { a[q] = r[q]; b[q] = a[q]; }
     } else {
         const int lb = (int)sb[key] - 1, rb = (int)sb[key + 1] - 1;
-        const double* r = base + (lb > 0 ? lb : 0) * st;
+        const int r0 = lb > 0 ? lb : 0, r1 = r0 + 1 < last ? r0 + 1 : last;
+        const double* ra = base + r0 * st;
+        const double* rb1 = base + r1 * st;
 #pragma unroll
-        for (int q = 0; q < kTeVals; ++q) v[q] = r[q];
+        for (int q = 0; q < kTeVals; ++q) { a[q] = ra[q]; b[q] = rb1[q]; }
         // the bucket's one grid point (E_{lb+1}) lies below E: the next interval
-        const bool up = rb > lb && (lb >= 0 ? v[1] : v[0]) < E;
+        const bool up = rb > lb && (lb >= 0 ? b[0] : a[0]) < E;
         i = up ? lb + 1 : lb;
         if (up && lb >= 0) {
-            const double* r2 = base + (int64_t)(lb + 1) * st;
+            const int r2 = lb + 2 < last ? lb + 2 : last;
+            const double* rc = base + r2 * st;
 #pragma unroll
-            for (int q = 0; q < kTeVals; ++q) v[q] = r2[q];
+            for (int q = 0; q < kTeVals; ++q) { a[q] = b[q]; b[q] = rc[q]; }
         }
     }
-    XsPoint P;
-    P.idx = i;
-    P.n = L.n;
-    if (i >= 0 && i < L.n - 1) {
-        P.a = (E - v[1]) / (v[0] - v[1]);
-        P.b = (E - v[0]) / (v[1] - v[0]);
+    const bool interp = i >= 0 && i < last;
+    double pa = 0.0, pb = 0.0;
+    if (interp) {
+        pa = (E - b[0]) / (a[0] - b[0]);
+        pb = (E - a[0]) / (b[0] - a[0]);
     }
     double t = 0.0;
 #pragma unroll
     for (int k = 0; k < kFastNuc; ++k)
-        if (k < L.nuc_count) t += xs_pair(v + 2 + 2 * k, P) * A.mat_dens[L.nuc_begin + k];
+        if (k < L.nuc_count) t += (interp ? pa * a[1 + k] + pb * b[1 + k] : a[1 + k]) * A.mat_dens[L.nuc_begin + k];
     idx = i;
     return t;
 }

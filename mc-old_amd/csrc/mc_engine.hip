@@ -28,7 +28,8 @@ template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
 __global__ void k_drain_close(Counters* c);
-__global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
+template <int kU> __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
+constexpr int kXsUnroll = 4;      // lookups in flight per thread in the standalone kernel (measured best of 1, 2, 4)
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
                             const int32_t* mats, int32_t nmat, uint64_t seed);
 __global__ void k_fill_f64(double* p, uint64_t n, double v);
@@ -409,20 +410,14 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
                 L.nuc_count = n;
                 L.nuc_begin = mat[m].nuc_begin;
                 if (!L.scaled) {
-                    // records with their grid energies for the LDS path
-                    L.te_stride = (2 + 2 * n + 7) / 8 * 8;
+                    // records (E_i, sigma_t[i] per nuclide) for the LDS path
+                    L.te_stride = (1 + n + 3) / 4 * 4;          // 32 or 64 bytes
                     L.te_off = (int64_t)rec_te.size();
                     rec_te.resize(rec_te.size() + (size_t)G.n * L.te_stride, 0.0);
                     for (int i = 0; i < G.n; ++i) {
-                        const int i2 = i + 1 < G.n ? i + 1 : i;
                         double* r = rec_te.data() + L.te_off + (int64_t)i * L.te_stride;
                         r[0] = grid_E[G.e_off + i];
-                        r[1] = grid_E[G.e_off + i2];
-                        for (int k = 0; k < n; ++k) {
-                            const int tb = me_tab[mat[m].nuc_begin + k];
-                            r[2 + 2 * k] = tab_t[tb + i];
-                            r[3 + 2 * k] = tab_t[tb + i2];
-                        }
+                        for (int k = 0; k < n; ++k) r[1 + k] = tab_t[me_tab[mat[m].nuc_begin + k] + i];
                     }
                 }
             }
@@ -993,8 +988,8 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     d->upload_static();
     if (d->lookup_lds() > 0) {
         // the standalone lookup stages up to kLdsBucketBytes of bucket table in LDS
-        HIP_CHECK(hipFuncSetAttribute((const void*)k_xs_lookup_array, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)d->lookup_lds()));
+        HIP_CHECK(hipFuncSetAttribute((const void*)k_xs_lookup_array<kXsUnroll>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->lookup_lds()));
     }
 }
 
@@ -1044,7 +1039,7 @@ void Engine::xs_lookup(const double* E, const int32_t* mat, double* out, uint64_
     HIP_CHECK(hipMemcpyAsync(bM.p, mat, 4 * n, hipMemcpyHostToDevice, d->stream));
     // problem 0's materials start at index 0
     const KArgs A = d->args(false, nullptr, 0);
-    hipLaunchKernelGGL(k_xs_lookup_array, d->lookup_grid(), d->lookup_block(), d->lookup_lds(), d->stream, A,
+    hipLaunchKernelGGL(k_xs_lookup_array<kXsUnroll>, d->lookup_grid(), d->lookup_block(), d->lookup_lds(), d->stream, A,
                        bE.as<double>(),
                        bM.as<int32_t>(), bO.as<double>(), n);
     HIP_CHECK(hipGetLastError());
@@ -1069,12 +1064,13 @@ void Engine::xs_bench(uint64_t n, int iters, uint64_t seed, double* sec, double*
     const KArgs A = d->args(false, nullptr, 0);
     const dim3 g = d->lookup_grid(), b = d->lookup_block();
     const size_t lds = d->lookup_lds();
-    hipLaunchKernelGGL(k_xs_lookup_array, g, b, lds, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
-                       bO.as<double>(), n);   // warm-up
-    HIP_CHECK(hipEventRecord(d->ev0, d->stream));
-    for (int k = 0; k < iters; ++k)
-        hipLaunchKernelGGL(k_xs_lookup_array, g, b, lds, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
+    auto launch = [&]() {
+        hipLaunchKernelGGL(k_xs_lookup_array<kXsUnroll>, g, b, lds, d->stream, A, bE.as<double>(), bM.as<int32_t>(),
                            bO.as<double>(), n);
+    };
+    launch();   // warm-up
+    HIP_CHECK(hipEventRecord(d->ev0, d->stream));
+    for (int k = 0; k < iters; ++k) launch();
     HIP_CHECK(hipEventRecord(d->ev1, d->stream));
     HIP_CHECK(hipEventSynchronize(d->ev1));
     HIP_CHECK(hipGetLastError());

@@ -1055,24 +1055,37 @@ __global__ void k_drain_close(Counters* c)
 }
 
 // Standalone lookup over arrays (mc_xs_lookup and its timing entry point).
+// kU independent lookups per thread per step; the input and output streams
+// bypass the caches' retention (non-temporal) so the tables stay in L2.
+template <int kU>
 __global__ void __launch_bounds__(1024) k_xs_lookup_array(KArgs A, const double* __restrict__ E,
                                                          const int32_t* __restrict__ mat,
                                                          double* __restrict__ out, uint64_t n)
 {
     const uint16_t* sb = stage_buckets(A, mc_lds_buckets);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
-        const uint64_t i2 = i + stride;
-        const bool two = i2 < n;
-        const int m = mat[i], m2 = two ? mat[i2] : -1;
-        const double e = E[i], e2 = two ? E[i2] : 1.0;
-        int g = 0, g2 = 0;
-        const double r = m >= 0 ? lookup_sigt(A, sb, m, e, g) : 0.0;
-        const double r2 = m2 >= 0 ? lookup_sigt(A, sb, m2, e2, g2) : 0.0;
-        out[i] = r;
-        if (two) out[i2] = r2;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kU * stride) {
+        int m[kU];
+        double e[kU], r[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t iu = i + u * stride;
+            m[u] = iu < n ? __builtin_nontemporal_load(mat + iu) : -1;
+            e[u] = iu < n ? __builtin_nontemporal_load(E + iu) : 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            int g = 0;
+            r[u] = m[u] >= 0 ? lookup_sigt(A, sb, m[u], e[u], g) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t iu = i + u * stride;
+            if (iu < n) __builtin_nontemporal_store(r[u], out + iu);
+        }
     }
 }
+template __global__ void k_xs_lookup_array<4>(KArgs, const double*, const int32_t*, double*, uint64_t);
 
 // Synthetic lookup inputs: E log-uniform in [emin, emax], material uniform.
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
