@@ -103,6 +103,14 @@ struct DParFold {
     double N, inv_theta;
 };
 
+// The same per nuclide entry, for materials where each nuclide has at most one
+// scatter-only and one capture-only parameter (mfold[m] = 1): problem-local
+// indices (-1: none) and 1/theta of the scatter parameter.
+struct DNucFold {
+    int32_t kl_s, kl_c;
+    double inv_theta_s;
+};
+
 // Everything a kernel needs, passed by value.
 struct KArgs {
     // problem data
@@ -131,6 +139,8 @@ struct KArgs {
     const mc_param_desc* param;   // material/nuc_slot rebased
     const double* theta;
     const DParFold* mpar;         // per mat_param entry (gradient runs)
+    const DNucFold* nfold;        // per material entry (gradient runs)
+    const int32_t* mfold;         // per material: 1 when nfold describes all its parameters
     const DNuc* nuc;
     const DGrid* grid;
     const double* grid_E;

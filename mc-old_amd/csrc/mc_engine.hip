@@ -118,7 +118,7 @@ struct Engine::Impl {
 
     // device data
     DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
-        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nuc, d_grid,
+        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nfold, d_mfold, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
         d_run_hb;
@@ -590,6 +590,8 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.param = d_param.as<mc_param_desc>();
     A.theta = d_theta.as<double>();
     A.mpar = d_mpar.as<DParFold>();
+    A.nfold = d_nfold.as<DNucFold>();
+    A.mfold = d_mfold.as<int32_t>();
     A.nuc = d_nuc.as<DNuc>();
     A.grid = d_grid.as<DGrid>();
     A.grid_E = d_grid_E.as<double>();
@@ -856,6 +858,26 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
             F.inv_theta = 1.0 / th[kg];
         }
     upload(d_mpar, mpar, stream);
+    std::vector<DNucFold> nfold(std::max<size_t>(1, mat_nuc.size()), DNucFold{-1, -1, 0.0});
+    std::vector<int32_t> mfold(std::max<size_t>(1, mat.size()), 0);
+    for (size_t m = 0; m < mat.size(); ++m) {
+        bool fast = mat[m].param_count > 0 && mat[m].nuc_count <= 8;
+        for (int q = 0; q < mat[m].param_count && fast; ++q) {
+            const DParFold& F = mpar[mat[m].param_begin + q];
+            DNucFold& N = nfold[mat[m].nuc_begin + F.slot];
+            if (F.reactions == MC_REACTION_SCATTER && N.kl_s < 0) {
+                N.kl_s = F.kl;
+                N.inv_theta_s = F.inv_theta;
+            } else if (F.reactions == MC_REACTION_CAPTURE && N.kl_c < 0) {
+                N.kl_c = F.kl;
+            } else {
+                fast = false;
+            }
+        }
+        mfold[m] = fast ? 1 : 0;
+    }
+    upload(d_nfold, nfold, stream);
+    upload(d_mfold, mfold, stream);
     HIP_CHECK(hipMemcpyAsync(d_run_hb.p, hb.data(), sizeof(uint64_t) * (np + 1), hipMemcpyHostToDevice, stream));
     if (grad) {
         std::vector<double> g(gout, gout + n_bins);

@@ -247,21 +247,15 @@ __device__ __forceinline__ void fold_one(const KArgs& A, uint32_t s, int kl, dou
 }
 
 // Surface crossing: fold the flight that ended on the surface, in material M
-// of the cell it leaves.  The parameters' cross sections are gathered first,
-// kChunk at a time, so their loads overlap.
+// of the cell it leaves (about one flight in a hundred in a shield, so one
+// parameter at a time, which keeps the flight kernel's registers low).
 __device__ void fold_surface(const KArgs& A, uint32_t s, const DProb& PR, const mc_material_desc& M, double E,
                              XsPoint& P, int gm)
 {
-    constexpr int kChunk = 4;
     const LrPending L = lr_pending(A, s);
-    for (int q0 = 0; q0 < M.param_count; q0 += kChunk) {
-        DSig d[kChunk];
-#pragma unroll
-        for (int t = 0; t < kChunk; ++t)
-            if (q0 + t < M.param_count) d[t] = param_dsig(A, M, q0 + t, E, P, gm);
-#pragma unroll
-        for (int t = 0; t < kChunk; ++t)
-            if (q0 + t < M.param_count) fold_one(A, s, d[t].k - PR.param_begin, d[t].ds, d[t].dc, 0.0, L);
+    for (int q = 0; q < M.param_count; ++q) {
+        const DSig d = param_dsig(A, M, q, E, P, gm);
+        fold_one(A, s, d.k - PR.param_begin, d.ds, d.dc, 0.0, L);
     }
     if (L.cs != 0.0 || L.cc != 0.0) {
         A.cs[s] = 0.0;
@@ -920,7 +914,28 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
             // dSigma_s/dtheta = sigma_s N and dSigma_c/dtheta = sigma_c N of the
             // parameter's entry: sigma_s from the registers above, sigma_c
             // read again (the record or table lines are in cache).
-            if (M.param_count > 0) {
+            if (M.param_count > 0 && A.mfold[m]) {
+                // per nuclide: its scatter and capture parameters, sigma_s from registers
+                const LrPending L = lr_pending(A, s);
+#pragma unroll
+                for (int k = 0; k < kNucReg; ++k) {
+                    if (k < M.nuc_count) {
+                        const int j = M.nuc_begin + k;
+                        const DNucFold F = A.nfold[j];
+                        const double N = A.mat_dens[j];
+                        if (F.kl_s >= 0)
+                            fold_one(A, s, F.kl_s, rs[k] * N, 0.0, sel == k ? F.inv_theta_s : 0.0, L);
+                        if (F.kl_c >= 0) {
+                            const double xs_c = rsc ? xs_pair(rsc + n2 + 2 * k, P) : entry_xs(A, j, XS_C, E, P, gm);
+                            fold_one(A, s, F.kl_c, 0.0, xs_c * N, 0.0, L);
+                        }
+                    }
+                }
+                if (L.cs != 0.0 || L.cc != 0.0) {
+                    A.cs[s] = 0.0;
+                    A.cc[s] = 0.0;
+                }
+            } else if (M.param_count > 0) {
                 const LrPending L = lr_pending(A, s);
                 for (int q = 0; q < M.param_count; ++q) {
                     const DParFold F = A.mpar[M.param_begin + q];
