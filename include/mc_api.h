@@ -189,8 +189,8 @@ typedef struct mc_engine_cfg {
 
 /* Per-kernel device time of the event loop, recorded when cfg.profile = 1:
  * kernels in order refill_count, refill_scatter, xs_lookup, flight (with the
- * surface crossings), event_scatter, collision, drain.                     */
-#define MC_N_KERNELS 7
+ * surface crossings), collision, drain.                                    */
+#define MC_N_KERNELS 6
 typedef struct mc_kernel_stats {
     uint64_t launches[MC_N_KERNELS];
     uint64_t items[MC_N_KERNELS];   /* particles (slots for refill) processed  */

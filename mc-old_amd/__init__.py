@@ -58,8 +58,7 @@ class TallyOut(C.Structure):
                 ("flights", C.c_uint64), ("seconds", C.c_double)]
 
 
-KERNEL_NAMES = ("k_refill_count", "k_refill_scatter", "k_xs_lookup", "k_flight", "k_event_scatter",
-                "k_collision", "k_drain")
+KERNEL_NAMES = ("k_refill_count", "k_refill_scatter", "k_xs_lookup", "k_flight", "k_collision", "k_drain")
 
 
 class KernelStats(C.Structure):

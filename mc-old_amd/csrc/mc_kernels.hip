@@ -13,14 +13,16 @@
 //                     40-83, 177-211); a flight that ends on a surface crosses
 //                     it here: boundary conditions, cell search, surface tallies,
 //                     importance, weight roulette (general.cpp:89-115,
-//                     population_control.cpp:9-49); collisions are marked and
-//                     counted per workgroup
-//   k_event_scatter   prefix over the counts -> collision queue C, in slot order
-//   k_collision       collision tallies, implicit capture, nuclide selection,
+//                     population_control.cpp:9-49); collisions go to the
+//                     workgroup's own segment of queue C
+//   k_collision       per workgroup, the segment its flight workgroup filled:
+//                     collision tallies, implicit capture, nuclide selection,
 //                     elastic scatter, weight roulette (general.cpp:121-163)
-// Queues are built without atomics: wave ballots rank lanes inside a block,
-// blocks rank among themselves from per-workgroup counts, so every queue keeps
-// slot order and consecutive lanes touch consecutive bank entries.
+// Queue A is built without atomics: wave ballots rank lanes inside a block,
+// blocks rank among themselves from per-workgroup counts, so it keeps slot
+// order and consecutive lanes touch consecutive bank entries.  Queue C is
+// block-local: a wave ballot ranks its lanes and one LDS atomic per wave
+// places the wave, so lanes stay in slot order within a wave.
 // A history that ends is closed out where it ends: the bins it touched are
 // added to the FP64 moment arrays (Estimator::end_history, Estimator.cpp:339-346).
 #include "mc_device.hpp"
@@ -740,71 +742,60 @@ template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
     constexpr uint32_t kXq = 2048;      // >= the largest workgroup
-    __shared__ uint32_t lds[32];
     __shared__ uint32_t xq[kXq];
-    __shared__ uint32_t xn;
+    __shared__ uint32_t xn, cn, sn;
     const uint32_t n = A.ctr->nA;
     uint32_t lo, hi;
     block_range(n, lo, hi);
-    if (threadIdx.x == 0) xn = 0;
+    if (threadIdx.x == 0) { xn = 0; cn = 0; sn = 0; }
     __syncthreads();
-    uint32_t nsurf = 0, ncoll = 0;
     uint32_t s_next = lo + threadIdx.x < hi ? A.qA[lo + threadIdx.x] : 0u;   // read one item ahead
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t s = s_next;
         if (i + blockDim.x < hi) s_next = A.qA[i + blockDim.x];
+        uint8_t st = ST_DEAD;
         if (i < hi) {
             bool crossed;
-            const uint8_t st = flight_one<kTables, kGrad, true>(A, s, crossed);
-            if (st == ST_COLL) {
-                ++ncoll;
-            } else if (st == ST_SURF) {
-                xq[atomicAdd(&xn, 1u)] = s;
-                ++nsurf;
-            }
+            st = flight_one<kTables, kGrad, true>(A, s, crossed);
         }
+        // collisions go to this block's segment of queue C (slots of a wave in
+        // lane order, one LDS atomic per wave); crossings to the LDS queue
+        const uint32_t ci = wave_append(st == ST_COLL, &cn);
+        if (st == ST_COLL) A.qC[lo + ci] = s;
+        const uint32_t xi = wave_append(st == ST_SURF, &xn);
+        if (st == ST_SURF) xq[xi] = s;
         __syncthreads();
         const uint32_t q = xn;          // the same in every thread between the barriers
         __syncthreads();
         if (q + blockDim.x > kXq || base + blockDim.x >= hi) {
             for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
             __syncthreads();
-            if (threadIdx.x == 0) xn = 0;
+            if (threadIdx.x == 0) { sn += q; xn = 0; }
             __syncthreads();
         }
     }
-    block_sum2(nsurf, ncoll, lds);
+    // the collision kernel's workgroup b takes this block's segment
     if (threadIdx.x == 0) {
-        A.wg_cnt[2 * blockIdx.x] = nsurf;
-        A.wg_cnt[2 * blockIdx.x + 1] = ncoll;
+        A.wg_cnt[2 * blockIdx.x] = sn;
+        A.wg_cnt[2 * blockIdx.x + 1] = cn;
     }
 }
 
-// Collision queue from the flight's marks, in A's order (the flight counted
-// its surface crossings, which it handled itself, beside its collisions).
-__global__ void __launch_bounds__(1024) k_event_scatter(KArgs A)
+// Event counts of an iteration (profiled runs only): the flight's per-workgroup
+// surface-crossing and collision counts summed into the counters.
+__global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* c)
 {
     __shared__ uint32_t lds[32];
-    const uint32_t n = A.ctr->nA;
-    uint32_t s_before, c_before, s_tot, c_tot;
-    wg_prefix(A.wg_cnt, s_before, c_before, s_tot, c_tot, lds);
-    (void)s_before;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A.ctr->nS = s_tot;
-        A.ctr->nC = c_tot;
+    uint32_t ns = 0, nc = 0;
+    for (uint32_t b = threadIdx.x; b < n_wg; b += blockDim.x) {
+        ns += wg_cnt[2 * b];
+        nc += wg_cnt[2 * b + 1];
     }
-    uint32_t lo, hi;
-    block_range(n, lo, hi);
-    uint32_t crun = c_before;
-    for (uint32_t base = lo; base < hi; base += blockDim.x) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t s = i < hi ? A.qA[i] : 0u;
-        const uint8_t st = i < hi ? A.status[s] : ST_DEAD;
-        uint32_t nc;
-        const uint32_t rc = block_flag_rank(st == ST_COLL, nc, lds);
-        if (st == ST_COLL) A.qC[crun + rc] = s;
-        crun += nc;
+    block_sum2(ns, nc, lds);
+    if (threadIdx.x == 0) {
+        c->nS = ns;
+        c->nC = nc;
     }
 }
 
@@ -1000,14 +991,17 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_collision(KArgs A)
 {
-    // the next queue entry is read one item ahead, off the item's load chain
-    const uint32_t n = A.ctr->nC;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t s_next = i < n ? A.qC[i] : 0u;
-    for (; i < n; i += stride) {
+    // workgroup b takes the collisions flight workgroup b queued (the same
+    // partition of queue A); the next entry is read one item ahead
+    uint32_t lo, hi;
+    block_range(A.ctr->nA, lo, hi);
+    const uint32_t n = A.wg_cnt[2 * blockIdx.x + 1];
+    const uint32_t* q = A.qC + lo;
+    uint32_t i = threadIdx.x;
+    uint32_t s_next = i < n ? q[i] : 0u;
+    for (; i < n; i += blockDim.x) {
         const uint32_t s = s_next;
-        if (i + stride < n) s_next = A.qC[i + stride];
+        if (i + blockDim.x < n) s_next = q[i + blockDim.x];
         collision_one<kTables, kGrad>(A, s);
     }
 }
