@@ -58,7 +58,15 @@ struct Counters {
     uint64_t flights;             // Ntrack
     uint32_t err_lost, err_split;
     uint64_t tA, tS, tC;          // particles processed by flight / surface / collision
+    uint32_t qa_ident, pad;       // 1: every slot below active_cap is live and queue A is
+                                  // the identity (not written; kernels use the slot index)
 };
+
+// Entry i of queue A.
+__device__ __forceinline__ uint32_t qa_at(const uint32_t* qA, uint32_t ident, uint32_t i)
+{
+    return ident ? i : qA[i];
+}
 
 constexpr int kFastNuc = 4;      // nuclides a record's fast path holds in registers
 

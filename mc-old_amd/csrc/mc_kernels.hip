@@ -480,10 +480,13 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
     const uint64_t remaining = end > next ? end - next : 0;
     uint32_t live_before, free_before, live_tot, free_tot;
     wg_prefix(A.wg_cnt, live_before, free_before, live_tot, free_tot, lds);
+    // every free slot gets a history: all slots live, queue A is the identity
+    const bool ident = (uint64_t)free_tot <= remaining && live_tot + free_tot == cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const uint64_t claim = (uint64_t)free_tot < remaining ? (uint64_t)free_tot : remaining;
         A.ctr->claimed = claim;
         A.ctr->nA = live_tot + (uint32_t)claim;
+        A.ctr->qa_ident = ident ? 1u : 0u;
     }
     uint32_t lo, hi;
     block_range(cap, lo, hi);
@@ -514,14 +517,16 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
     for (uint32_t i = a; i < b; ++i) {
         const uint8_t st = A.status[i];
         if (st == ST_ALIVE) {
-            A.qA[act++] = i;
+            if (!ident) A.qA[act] = i;
+            ++act;
         } else if (st == ST_DEAD || st == ST_EMPTY) {
             if (fidx < remaining) {
                 const uint64_t hid = next + fidx;
                 while (p + 1 < A.n_prob && A.run_hb[p + 1] <= hid) ++p;
                 const uint64_t js = history_seed(j0, hid - h0, A.jump);
                 start_history<kGrad>(A, i, hid, p, js);
-                A.qA[act++] = i;
+                if (!ident) A.qA[act] = i;
+                ++act;
             } else if (st != ST_EMPTY) {
                 A.status[i] = ST_EMPTY;
             }
@@ -540,12 +545,12 @@ __global__ void __launch_bounds__(1024) k_xs_lookup(KArgs A, int use_lds)
 {
     const uint16_t* sb = use_lds ? stage_buckets(A, mc_lds_buckets) : nullptr;
     // two independent lookups per thread per step keep more gathers in flight
-    const uint32_t n = A.ctr->nA;
+    const uint32_t n = A.ctr->nA, ident = A.ctr->qa_ident;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
         const uint32_t i2 = i + stride;
-        const uint32_t s = A.qA[i];
-        const uint32_t s2 = i2 < n ? A.qA[i2] : s;
+        const uint32_t s = qa_at(A.qA, ident, i);
+        const uint32_t s2 = i2 < n ? qa_at(A.qA, ident, i2) : s;
         const int m = A.pmat[s], m2 = A.pmat[s2];
         const double E = A.E[s], E2 = A.E[s2];
         int g = 0, g2 = 0;
@@ -749,11 +754,12 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     block_range(n, lo, hi);
     if (threadIdx.x == 0) { xn = 0; cn = 0; sn = 0; }
     __syncthreads();
-    uint32_t s_next = lo + threadIdx.x < hi ? A.qA[lo + threadIdx.x] : 0u;   // read one item ahead
+    const uint32_t ident = A.ctr->qa_ident;
+    uint32_t s_next = lo + threadIdx.x < hi ? qa_at(A.qA, ident, lo + threadIdx.x) : 0u;   // read one item ahead
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t s = s_next;
-        if (i + blockDim.x < hi) s_next = A.qA[i + blockDim.x];
+        if (i + blockDim.x < hi) s_next = qa_at(A.qA, ident, i + blockDim.x);
         uint8_t st = ST_DEAD;
         if (i < hi) {
             bool crossed;
@@ -1021,7 +1027,7 @@ __global__ void __launch_bounds__(1024) k_drain(KArgs A, uint32_t max_events)
     uint32_t nfl = 0, nsurf = 0, ncoll = 0;
     MC_GRID_LOOP(n)
     {
-        const uint32_t s = A.qA[_i];
+        const uint32_t s = qa_at(A.qA, A.ctr->qa_ident, _i);
         for (uint32_t e = 0; e < max_events; ++e) {
             if (kTables) {
                 const int m = A.pmat[s];
