@@ -24,7 +24,10 @@ constexpr int kBucketBits = 10;                // buckets per octave = 1024
 constexpr int kBucketShift = 52 - kBucketBits;
 
 // slot status; flight marks the next event, surface/collision resolve it
-enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4 };
+// ST_HELD: a member of a split family ended here; the slot has no particle,
+// but its tally row is a family's row while fcount[slot] > 0 (then only split
+// copies, which score into their own family's row, may take the slot).
+enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4, ST_HELD = 5 };
 
 constexpr int kTouched = 8;        // per-slot list of tally bins touched by the history
 
@@ -60,6 +63,8 @@ struct Counters {
     uint64_t tA, tS, tC;          // particles processed by flight / surface / collision
     uint32_t qa_ident, pad;       // 1: every slot below active_cap is live and queue A is
                                   // the identity (not written; kernels use the slot index)
+    uint64_t chead, ctail;        // copy ring: split copies waiting for a slot are [chead, ctail)
+    uint32_t cplaced, err_ring;   // copies the last refill placed; ring overflows
 };
 
 // Entry i of queue A.
@@ -165,6 +170,9 @@ struct KArgs {
     const double* src_cdf;        // per problem: src_count + 1 entries at cdf_off
     const int32_t* cdf_off;
     const uint64_t* jump;         // 64 LCG jump multipliers
+    uint64_t jump40;              // g^(2^40): a split copy's stream starts (i + 1) 2^40 draws on
+    int32_t split_any;            // some problem can split (importance rises between cells)
+    int32_t pad_split;
     const uint64_t* run_hb;       // per problem first key of this run (n_prob + 1)
     int32_t n_prob;
     int32_t bin_stride;           // hist row length (max bins per problem)
@@ -182,6 +190,17 @@ struct KArgs {
     uint8_t* tcount;              // touched-bin count per slot (kTouched + 1: overflow)
     uint32_t* tbins;              // [slot][kTouched] touched bins (problem-local)
     uint32_t* wg_cnt;             // per-workgroup counts for the two-pass compaction
+    // split families (when split_any): per slot the family of its particle, as
+    // the slot whose tally row the family shares (-1: none), and per row slot the
+    // family's members not yet ended (copies in the ring included)
+    int32_t *fam, *fcount;
+    uint32_t* wg_held;            // per workgroup: ended slots whose row a family holds
+    // copy ring (capacity ccap): particles made by importance splitting
+    double *cx, *cy, *cz, *cu, *cv, *cw, *cE, *cwgt;
+    uint64_t *crng, *chid;
+    int32_t *ccell, *cprob, *cfam;
+    uint32_t* cnfl;
+    uint64_t ccap;
     double* hist;
     // likelihood-ratio state: score function S and H = sum of dS * Q, parameter-
     // major [param][slot] with row stride cap,

@@ -35,6 +35,7 @@ __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, d
 __global__ void k_fill_f64(double* p, uint64_t n, double v);
 __global__ void k_reduce_rows(const double* part, uint64_t rows, uint64_t n, double* out);
 __global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v);
+__global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v);
 __global__ void k_selftest_math(const double* in, double* out, uint64_t n);
 
 #define HIP_CHECK(x)                                                                          \
@@ -114,6 +115,7 @@ struct Engine::Impl {
     std::vector<uint64_t> jump;
     int n_bins = 0, n_params = 0, bin_stride = 1, par_stride = 1;
     bool has_tables = false;
+    bool split_any = false;            // some problem's cells differ in importance
     double emin = 0, emax = 0;
 
     // device data
@@ -125,6 +127,10 @@ struct Engine::Impl {
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
         b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, q_A, q_C, d_ctr, d_wgcnt, b_mat;
+    // split families and the copy ring (problems that can split only)
+    DevBuf b_fam, b_fcount, d_wgheld, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
+        r_nfl;
+    uint64_t ring_cap = 0;
     uint32_t h_active_cap = 0;
     uint64_t bank_cap = 0;
     bool bank_grad = false;
@@ -245,6 +251,15 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
             S.est_begin += att_off;
             surf.push_back(S);
         }
+        // a crossing from a cell of importance I > 0 into one of higher
+        // importance splits the particle (importance 0 only kills)
+        double imin = 0.0, imax = 0.0;
+        for (int c = 0; c < d.n_cells; ++c) {
+            const double I = d.cells[c].importance;
+            if (I > 0.0 && (imin == 0.0 || I < imin)) imin = I;
+            imax = std::max(imax, I);
+        }
+        if (imin > 0.0 && imax > imin) split_any = true;
         for (int c = 0; c < d.n_cells; ++c) {
             mc_cell_desc C = d.cells[c];
             if (C.material >= 0) C.material += mat_o;
@@ -498,6 +513,7 @@ void Engine::Impl::upload_static()
     upload(d_src_cdf, src_cdf, stream);
     upload(d_cdf_off, cdf_off, stream);
     upload(d_jump, jump, stream);
+    if (split_any) d_wgheld.alloc(sizeof(uint32_t) * 8192);
     d_run_hb.alloc(sizeof(uint64_t) * (prob.size() + 1));
     t_sum.alloc(sizeof(double) * std::max(1, n_bins));
     t_sq.alloc(sizeof(double) * std::max(1, n_bins));
@@ -508,6 +524,8 @@ void Engine::Impl::upload_static()
     d_wgcnt.alloc(sizeof(uint32_t) * 2 * 8192);   // per-workgroup counts (grid <= 8192 blocks)
     HIP_CHECK(hipStreamSynchronize(stream));
 }
+
+constexpr uint64_t kRingPerSlot = 8;
 
 void Engine::Impl::ensure_bank(bool grad)
 {
@@ -530,6 +548,15 @@ void Engine::Impl::ensure_bank(bool grad)
         b_hist.alloc(8 * B * (uint64_t)bin_stride);
         q_A.alloc(4 * B);
         q_C.alloc(4 * B);
+        if (split_any) {
+            // copies wait in the ring while no slot is free: 8 per bank slot
+            const uint64_t R = kRingPerSlot * B;
+            b_fam.alloc(4 * B);
+            b_fcount.alloc(4 * B);
+            for (DevBuf* r : {&r_x, &r_y, &r_z, &r_u, &r_v, &r_w, &r_E, &r_wgt, &r_rng, &r_hid}) r->alloc(8 * R);
+            for (DevBuf* r : {&r_cell, &r_prob, &r_fam, &r_nfl}) r->alloc(4 * R);
+            ring_cap = R;
+        }
         bank_cap = B;
         bank_grad = false;
     }
@@ -608,6 +635,24 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.src_cdf = d_src_cdf.as<double>();
     A.cdf_off = d_cdf_off.as<int32_t>();
     A.jump = d_jump.as<uint64_t>();
+    A.jump40 = lcg_pow(kRnMult, 1ull << 40);
+    A.split_any = split_any ? 1 : 0;
+    if (split_any) {
+        A.fam = b_fam.as<int32_t>();
+        A.fcount = b_fcount.as<int32_t>();
+        A.wg_held = d_wgheld.as<uint32_t>();
+        A.cx = r_x.as<double>(); A.cy = r_y.as<double>(); A.cz = r_z.as<double>();
+        A.cu = r_u.as<double>(); A.cv = r_v.as<double>(); A.cw = r_w.as<double>();
+        A.cE = r_E.as<double>();
+        A.cwgt = r_wgt.as<double>();
+        A.crng = r_rng.as<uint64_t>();
+        A.chid = r_hid.as<uint64_t>();
+        A.ccell = r_cell.as<int32_t>();
+        A.cprob = r_prob.as<int32_t>();
+        A.cfam = r_fam.as<int32_t>();
+        A.cnfl = r_nfl.as<uint32_t>();
+        A.ccap = ring_cap;
+    }
     A.run_hb = d_run_hb.as<uint64_t>();
     A.n_prob = (int32_t)prob.size();
     A.bin_stride = bin_stride;
@@ -906,7 +951,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     HIP_CHECK(hipEventRecord(ev0, stream));
     const bool plain = run && run->mode == MC_RUN_PLAIN;
     uint64_t flights = 0;
-    uint32_t err_lost = 0, err_split = 0;
+    uint32_t err_lost = 0, err_split = 0, err_ring = 0;
     for (int p = 0; p < (plain ? np : 1); ++p) {
         Counters c{};
         c.next_hist = plain ? hb[p] : 0;
@@ -916,10 +961,15 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         h_active_cap = c.active_cap;
         HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.status, bank_cap, (uint8_t)ST_EMPTY);
+        if (split_any) {
+            hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, stream, A.fam, bank_cap, -1);
+            hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, stream, A.fcount, bank_cap, 0);
+        }
         run_loop(grad, A);
         flights += h_ctr->tA;      // every entry of queue A is one flight
         err_lost += h_ctr->err_lost;
         err_split += h_ctr->err_split;
+        err_ring += h_ctr->err_ring;
     }
     HIP_CHECK(hipEventRecord(ev1, stream));
     HIP_CHECK(hipEventSynchronize(ev1));
@@ -927,7 +977,10 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
     if (err_lost) throw Error(MC_ERR_LOST_PARTICLE, std::to_string(err_lost) + " particle(s) lost");
     if (err_split)
-        throw Error(MC_ERR_UNSUPPORTED, "importance splitting (ratio > 1) is not supported by the GPU engine");
+        throw Error(MC_ERR_UNSUPPORTED, "importance splitting (ratio > 1) is not supported in gradient runs");
+    if (err_ring)
+        throw Error(MC_ERR_CAPACITY, "split copies overflowed the copy ring (" + std::to_string(ring_cap) +
+                                         " entries, 8 per bank slot); raise bank_capacity");
     // reduce the per-workgroup partial moments (workgroups in order)
     const dim3 rg(256), rb(256);
     hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.psum, part_rows, (uint64_t)n_bins, t_sum.as<double>());

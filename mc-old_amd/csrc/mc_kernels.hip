@@ -140,6 +140,15 @@ __device__ __forceinline__ int find_cell(const KArgs& A, int s, const DProb& PR,
 // first time it becomes non-zero.
 __device__ __forceinline__ void hist_add(const KArgs& A, uint32_t s, int b, double v)
 {
+    if (A.split_any) {
+        // a member of a split family scores into the family's row (shared, so atomic;
+        // the family closes out by a full scan of its row)
+        const int f = A.fam[s];
+        if (f >= 0) {
+            if (v != 0.0) atomicAdd(A.hist + (size_t)f * (size_t)A.bin_stride + b, v);
+            return;
+        }
+    }
     double* h = A.hist + (size_t)s * (size_t)A.bin_stride + b;
     const double old = *h;
     *h = old + v;
@@ -277,12 +286,31 @@ __device__ __forceinline__ void close_bin(const KArgs& A, double* row, const DPr
     }
 }
 
+// A member of a split family ends: the last one closes out the family's row
+// (Estimator::end_history for the whole history, parent and copies).  The
+// slot's status becomes ST_HELD: its row may still be a family's.
+__device__ uint8_t finish_family(const KArgs& A, uint32_t s, const DProb& PR)
+{
+    const int f = A.fam[s];
+    A.fam[s] = -1;
+    __threadfence();                              // this member's scores before the count
+    if (atomicSub(&A.fcount[f], 1) == 1) {
+        __threadfence();                          // every member's scores after it
+        double* row = A.hist + (size_t)f * (size_t)A.bin_stride;
+        for (int b = 0; b < PR.bin_count; ++b) close_bin(A, row, PR, b);
+        A.tcount[f] = 0;
+    }
+    return ST_HELD;
+}
+
 // End of a history: close out its tally row (Estimator::end_history).
+// Returns the status the caller stores in slot s.
 // kFlush: scores may still be pending (not after a collision, which folds them).
 template <bool kGrad, bool kFlush>
-__device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
-                               double E, uint32_t nfl)
+__device__ uint8_t finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
+                                  double E, uint32_t nfl)
 {
+    if (A.split_any && A.fam[s] >= 0) return finish_family(A, s, PR);
     double* row = A.hist + (size_t)s * (size_t)A.bin_stride;
     const uint32_t nt = A.tcount[s];
     const uint32_t* tb = A.tbins + (size_t)s * kTouched;
@@ -351,6 +379,7 @@ __device__ void finish_history(const KArgs& A, uint32_t s, int p, const DProb& P
             }
         }
     }
+    return ST_DEAD;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -368,21 +397,32 @@ __global__ void __launch_bounds__(1024) k_refill_count(KArgs A)
         c->tC += c->nC;
         c->next_hist += c->claimed;
         c->claimed = 0;
+        c->chead += c->cplaced;    // copies the last refill placed left the ring
+        c->cplaced = 0;
         c->nS = 0;
         c->nC = 0;
     }
     uint32_t lo, hi;
     block_range(A.ctr->active_cap, lo, hi);
-    uint32_t live = 0, freec = 0;
+    uint32_t live = 0, freec = 0, held = 0;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
         const uint8_t st = A.status[i];
         live += st == ST_ALIVE;
         freec += st == ST_DEAD || st == ST_EMPTY;
+        if (st == ST_HELD) {
+            if (A.fcount[i] > 0) ++held;
+            else ++freec;
+        }
     }
     block_sum2(live, freec, lds);
+    if (A.split_any) {
+        uint32_t zero = 0;
+        block_sum2(held, zero, lds);
+    }
     if (threadIdx.x == 0) {
         A.wg_cnt[2 * blockIdx.x] = live;
         A.wg_cnt[2 * blockIdx.x + 1] = freec;
+        if (A.split_any) A.wg_held[blockIdx.x] = held;
     }
 }
 
@@ -439,6 +479,25 @@ __device__ __forceinline__ void start_history(const KArgs& A, uint32_t i, uint64
     }
 }
 
+// A split copy from ring entry k takes slot i (its family's row stays where
+// the family began; the slot's own row is not touched).
+__device__ __forceinline__ void start_copy(const KArgs& A, uint32_t i, uint64_t k)
+{
+    const int c = A.ccell[k];
+    A.x[i] = A.cx[k]; A.y[i] = A.cy[k]; A.z[i] = A.cz[k];
+    A.u[i] = A.cu[k]; A.v[i] = A.cv[k]; A.w[i] = A.cw[k];
+    A.E[i] = A.cE[k];
+    A.wgt[i] = A.cwgt[k];
+    A.rng[i] = A.crng[k];
+    A.hid[i] = A.chid[k];
+    A.pcell[i] = c;
+    A.pmat[i] = A.cell[c].material;
+    A.pprob[i] = A.cprob[k];
+    A.nflight[i] = A.cnfl[k];
+    A.fam[i] = A.cfam[k];
+    A.status[i] = ST_ALIVE;
+}
+
 // Block-wide exclusive prefix of two counts per thread (wave shuffles, then
 // the wave totals through LDS); also returns the block totals.
 __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& a_ex, uint32_t& b_ex, uint32_t& a_tot,
@@ -478,14 +537,33 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
     const uint32_t cap = A.ctr->active_cap;
     const uint64_t next = A.ctr->next_hist, end = A.ctr->end_hist;
     const uint64_t remaining = end > next ? end - next : 0;
+    // split copies waiting in the ring go first: a refill that has any places
+    // copies only, into free slots and slots whose row a family holds
+    const uint64_t chead = A.ctr->chead;
+    const uint64_t pend = A.split_any && A.ctr->err_ring == 0 ? A.ctr->ctail - chead : 0;
+    const bool cmode = pend > 0;
     uint32_t live_before, free_before, live_tot, free_tot;
     wg_prefix(A.wg_cnt, live_before, free_before, live_tot, free_tot, lds);
+    if (cmode) {
+        uint32_t hb = 0, ht = 0;
+        for (uint32_t v = threadIdx.x; v < gridDim.x; v += blockDim.x) {
+            const uint32_t h = A.wg_held[v];
+            if (v < blockIdx.x) hb += h;
+            ht += h;
+        }
+        block_sum2(hb, ht, lds);
+        free_before += hb;
+        free_tot += ht;
+    }
+    // lim: free slots (in rank order) that get a particle
+    const uint64_t lim = cmode ? (pend < free_tot ? pend : (uint64_t)free_tot)
+                               : ((uint64_t)free_tot < remaining ? (uint64_t)free_tot : remaining);
     // every free slot gets a history: all slots live, queue A is the identity
-    const bool ident = (uint64_t)free_tot <= remaining && live_tot + free_tot == cap;
+    const bool ident = !cmode && (uint64_t)free_tot <= remaining && live_tot + free_tot == cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const uint64_t claim = (uint64_t)free_tot < remaining ? (uint64_t)free_tot : remaining;
-        A.ctr->claimed = claim;
-        A.ctr->nA = live_tot + (uint32_t)claim;
+        A.ctr->claimed = cmode ? 0 : lim;
+        A.ctr->cplaced = cmode ? (uint32_t)lim : 0u;
+        A.ctr->nA = live_tot + (uint32_t)lim;
         A.ctr->qa_ident = ident ? 1u : 0u;
     }
     uint32_t lo, hi;
@@ -496,14 +574,14 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
     for (uint32_t i = a; i < b; ++i) {
         const uint8_t st = A.status[i];
         nlive += st == ST_ALIVE;
-        nfree += st == ST_DEAD || st == ST_EMPTY;
+        nfree += st == ST_DEAD || st == ST_EMPTY || (st == ST_HELD && (cmode || A.fcount[i] == 0));
     }
     uint32_t live_ex, free_ex, lt, ft;
     block_scan2(nlive, nfree, live_ex, free_ex, lt, ft, lds);
     uint64_t fidx = (uint64_t)free_before + free_ex;          // rank of my next free slot
-    uint64_t act = (uint64_t)live_before + live_ex + (fidx < remaining ? fidx : remaining);
+    uint64_t act = (uint64_t)live_before + live_ex + (fidx < lim ? fidx : lim);
     // the wave's first new key: the smallest fidx of a lane that starts one
-    const bool starts = nfree > 0 && fidx < remaining;
+    const bool starts = !cmode && nfree > 0 && fidx < lim;
     const uint64_t mask = __ballot(starts);
     uint64_t h0 = 0, j0 = 1;
     int p0 = 0;
@@ -519,15 +597,19 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
         if (st == ST_ALIVE) {
             if (!ident) A.qA[act] = i;
             ++act;
-        } else if (st == ST_DEAD || st == ST_EMPTY) {
-            if (fidx < remaining) {
-                const uint64_t hid = next + fidx;
-                while (p + 1 < A.n_prob && A.run_hb[p + 1] <= hid) ++p;
-                const uint64_t js = history_seed(j0, hid - h0, A.jump);
-                start_history<kGrad>(A, i, hid, p, js);
+        } else if (st == ST_DEAD || st == ST_EMPTY || (st == ST_HELD && (cmode || A.fcount[i] == 0))) {
+            if (fidx < lim) {
+                if (cmode) {
+                    start_copy(A, i, (chead + fidx) % A.ccap);
+                } else {
+                    const uint64_t hid = next + fidx;
+                    while (p + 1 < A.n_prob && A.run_hb[p + 1] <= hid) ++p;
+                    const uint64_t js = history_seed(j0, hid - h0, A.jump);
+                    start_history<kGrad>(A, i, hid, p, js);
+                }
                 if (!ident) A.qA[act] = i;
                 ++act;
-            } else if (st != ST_EMPTY) {
+            } else if (st == ST_DEAD) {
                 A.status[i] = ST_EMPTY;
             }
             ++fidx;
@@ -580,6 +662,53 @@ __device__ __forceinline__ void weight_roulette(double& w, bool& alive, uint64_t
 // p.  Writes the particle back and returns its new status.  Kept out of line:
 // about one flight in a hundred crosses a surface in a shield, and the flight
 // kernel's registers stay for the common path.
+// Importance splitting into n (population_control.cpp:21-49): the particle
+// goes on as one member of its history's split family and n - 1 copies of it
+// wait in the copy ring for a slot; copy i (from 0) starts (i + 1) 2^40 draws
+// ahead of the particle's stream.  The family's members score into the tally
+// row of the slot where the family began, and the last member to end closes
+// it out.  Gradient runs do not split (the count reports it).
+template <bool kGrad>
+__device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, Vec3 p, Vec3 u, double E, double w,
+                                          uint64_t rng, uint32_t nfl, int cell, int p_id)
+{
+    if (kGrad || !A.split_any) {
+        atomicAdd(&A.ctr->err_split, 1u);
+        return;
+    }
+    const uint32_t nc = (uint32_t)(n - 1);
+    const uint64_t t0 = atomicAdd((unsigned long long*)&A.ctr->ctail, (unsigned long long)nc);
+    if (t0 + nc - A.ctr->chead > A.ccap) {
+        // no room: the run reports the overflow and places no further copies
+        atomicAdd(&A.ctr->err_ring, 1u);
+        return;
+    }
+    int f = A.fam[s];
+    if (f < 0) {
+        f = (int)s;
+        A.fam[s] = f;
+        atomicAdd(&A.fcount[f], n);          // the particle and its copies
+    } else {
+        atomicAdd(&A.fcount[f], (int)nc);
+    }
+    const uint64_t hid = A.hid[s];
+    uint64_t r = rng;
+    for (uint32_t i = 0; i < nc; ++i) {
+        r = (r * A.jump40) & kRnMask;
+        const uint64_t k = (t0 + i) % A.ccap;
+        A.cx[k] = p.x; A.cy[k] = p.y; A.cz[k] = p.z;
+        A.cu[k] = u.x; A.cv[k] = u.y; A.cw[k] = u.z;
+        A.cE[k] = E;
+        A.cwgt[k] = w;
+        A.crng[k] = r;
+        A.chid[k] = hid;
+        A.ccell[k] = cell;
+        A.cprob[k] = p_id;
+        A.cfam[k] = f;
+        A.cnfl[k] = nfl;
+    }
+}
+
 template <bool kGrad>
 __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, int sf, Vec3 p, Vec3 u, double E,
                                                  double w, uint64_t rng, uint32_t nfl, int cell_old, int p_id)
@@ -634,7 +763,7 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
         } else {
             const int nsplit = (int)floor(rat + urand(rng));
             w = w / (double)nsplit;
-            if (nsplit > 1) atomicAdd(&A.ctr->err_split, 1u);   // splitting: unsupported here
+            if (nsplit > 1) spawn_copies<kGrad>(A, s, nsplit, p, u, E, w, rng, nfl, cell, p_id);
         }
     }
     weight_roulette(w, alive, rng, PR);
@@ -646,8 +775,7 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
     A.pcell[s] = cell;
     A.pmat[s] = A.cell[cell].material;
     if (alive) return ST_ALIVE;
-    finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, nfl);
-    return ST_DEAD;
+    return finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, nfl);
 }
 
 // One flight of slot s: distance to collision and to the nearest surface,
@@ -721,8 +849,7 @@ __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& 
         A.rng[s] = rng;
         A.nflight[s] = nfl;
         A.wgt[s] = 0.0;
-        finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, nfl);
-        st = ST_DEAD;
+        st = finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, nfl);
     }
     A.status[s] = st;
     return st;
@@ -988,8 +1115,7 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
         return ST_ALIVE;
     } else {
         const Vec3 p = {A.x[s], A.y[s], A.z[s]};
-        finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
-        A.status[s] = ST_DEAD;
+        A.status[s] = finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
     }
     return ST_DEAD;
 }
@@ -1144,6 +1270,12 @@ __global__ void k_fill_f64(double* p, uint64_t n, double v)
 }
 
 __global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;

@@ -160,3 +160,13 @@ def shield_xs_dir() -> str:
     if not os.path.exists(marker):
         decks.write_shield_nuclides(_SHIELD_XS)
     return _SHIELD_XS
+
+
+def importance_slab(L: float, c: float, nb: int, n: int, ratio: float) -> str:
+    """One-speed slab whose flux bin k has importance ratio**k: every crossing
+    deeper splits (ratio > 1), every crossing back plays roulette."""
+    deck = decks.slab_deck(L, c, nb, n)
+    for k in range(1, nb):
+        deck = deck.replace(f'<cell name="bin{k}" material="slab">',
+                            f'<cell name="bin{k}" material="slab" importance="{ratio ** k!r}">')
+    return deck

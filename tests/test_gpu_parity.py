@@ -121,10 +121,10 @@ def test_batched_equals_plain_equals_oracle():
 
 
 def test_error_paths_on_device():
-    # importance splitting is rejected by the GPU engine with a status code
-    xml = decks.slab_deck(2.0, 0.5, 2, 100).replace('<cell name="bin1" material="slab">',
-                                                    '<cell name="bin1" material="slab" importance="4.0">')
-    p = mc.Problem.from_string(xml)
+    # a copy ring that overflows (importance x8 per bin, 64 bank slots) ends the
+    # run with a capacity status rather than losing copies silently
+    from mc_testlib import importance_slab
+    p = mc.Problem.from_string(importance_slab(4.0, 0.5, 8, 2000, 8.0))
     with pytest.raises(mc.MCError) as e:
-        mc.Engine(p).forward()
-    assert e.value.status == mc.MC_ERR_UNSUPPORTED
+        mc.Engine(p, bank_capacity=64).forward()
+    assert e.value.status == mc.MC_ERR_CAPACITY

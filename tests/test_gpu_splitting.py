@@ -1,0 +1,85 @@
+"""GPU importance splitting (population_control.cpp:21-49) against the oracle.
+
+Each particle's walk is bit-identical to the oracle's (copy i of a split
+starts (i + 1) 2^40 draws ahead of the parent's stream), so the total flight
+count matches exactly.  A split family shares one tally row and its members
+add to it with atomics in whatever order they run, so per-history sums differ
+from the oracle's by rounding only: moment sums agree to 1e-12 relative.
+A full-size run with importances rising through the slab is checked against
+the deterministic slab solution within 3 sigma.
+"""
+import numpy as np
+import pytest
+
+import slab_analytic as sa
+from mc_testlib import decks, importance_slab, mc, oracle_run
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(p, n, **kw):
+    g = mc.Engine(p, **kw).forward()
+    o = oracle_run(p, n)
+    assert g.flights == o.flights
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(g.sq, o.sq, rtol=1e-12, atol=1e-300)
+    return g, o
+
+
+# drain_divisor 1: the run's tail goes to the drain kernel; -1: event iterations only
+@pytest.mark.parametrize("drain", [1, -1])
+def test_split_doubling_matches_oracle(drain):
+    """Importance doubles per bin: every deeper crossing splits in two."""
+    n = 20000
+    p = mc.Problem.from_string(importance_slab(4.0, 0.5, 8, n, 2.0))
+    g, o = _check(p, n, bank_capacity=1 << 14, drain_divisor=drain)
+    assert o.flights > 5 * n       # the deck does split (12 flights per history without)
+
+
+def test_split_fractional_ratio_small_bank():
+    """Ratio 1.5 (splits into one or two), strong scattering, and a bank of
+    1024 slots: copies wait in the ring and take slots whose row a family
+    still holds."""
+    n = 5000
+    p = mc.Problem.from_string(importance_slab(4.0, 0.9, 8, n, 1.5))
+    _check(p, n, bank_capacity=1 << 10, iters_per_graph=5)
+
+
+def test_split_plain_launches_and_batch():
+    """The same deck through ordinary launches, and batched with a deck that
+    does not split (rows of both kinds in one bank)."""
+    n = 4000
+    ps = [mc.Problem.from_string(importance_slab(3.0, 0.7, 6, n, 2.0)),
+          mc.Problem.from_string(decks.slab_deck(3.0, 0.7, 6, n))]
+    _check(ps[0], n, bank_capacity=1 << 12, use_graph=False)
+    eng = mc.Engine(ps, bank_capacity=1 << 12)
+    b = eng.forward()
+    off = 0
+    for i, p in enumerate(ps):
+        o = oracle_run(p, n, key_offset=off)
+        np.testing.assert_allclose(b.sum[eng.bin_range(i)], o.sum, rtol=1e-12, atol=1e-300)
+        np.testing.assert_allclose(b.sq[eng.bin_range(i)], o.sq, rtol=1e-12, atol=1e-300)
+        off += n
+
+
+def test_split_full_size_vs_analytic():
+    """1e6 histories of the 4 cm, c = 0.5 slab with importance 1.03^k in flux
+    bin k: T, R and the flux bins within 3 sigma of the deterministic slab."""
+    n = 1000000
+    p = mc.Problem.from_string(importance_slab(4.0, 0.5, 100, n, 1.03))
+    g = mc.Engine(p).forward()
+    a = sa.slab(4.0, 0.5)
+    assert abs(g.mean[0] - a["T"]) < 3 * g.uncer[0]
+    assert abs(g.mean[1] - a["R"]) < 3 * g.uncer[1]
+    z = (g.mean[2:] - a["flux"]) / g.uncer[2:]
+    assert np.mean(np.abs(z) > 3) < 0.03 and np.abs(z).max() < 4.5
+
+
+def test_split_in_gradient_run_is_rejected():
+    n = 1000
+    deck = decks.slab_deck(2.0, 0.5, 2, n, params=("scatter",))
+    deck = deck.replace('<cell name="bin1" material="slab">', '<cell name="bin1" material="slab" importance="4.0">')
+    p = mc.Problem.from_string(deck)
+    with pytest.raises(mc.MCError) as e:
+        mc.Engine(p).backward(np.ones(p.n_bins))
+    assert e.value.status == mc.MC_ERR_UNSUPPORTED
