@@ -48,6 +48,7 @@ struct DProb {
     int32_t src_begin, src_count;
     int32_t bin_begin, bin_count;
     int32_t param_begin, param_count;
+    int32_t mat_begin, mat_count;
     uint64_t seed;
     double wr, ws;
 };
@@ -209,6 +210,11 @@ struct KArgs {
     double* S;
     double* H;
     double *Qw, *cs, *cc, *lpend;
+    // per slot: the materials with parameters the history has been in (bit
+    // m - mat_begin; bit 31: a material of index 31 or above), whose S and H
+    // entries its closeout reads and clears
+    uint32_t* vmat;
+    uint32_t* qS;                 // gradient runs: surface crossings, per flight workgroup segment
     uint32_t *qA, *qC;
     Counters* ctr;
     // tallies: per-workgroup partial moments [workgroup][bin], reduced at the

@@ -26,6 +26,7 @@ __global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* 
 __global__ void k_xs_lookup(KArgs A, int use_lds);
 template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
+template <bool kTables> __global__ void k_cross(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
 __global__ void k_drain_close(Counters* c);
 template <int kU> __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
@@ -126,7 +127,7 @@ struct Engine::Impl {
         d_run_hb;
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
-        b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, q_A, q_C, d_ctr, d_wgcnt, b_mat;
+        b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, b_vmat, q_S, q_A, q_C, d_ctr, d_wgcnt, b_mat;
     // split families and the copy ring (problems that can split only)
     DevBuf b_fam, b_fcount, d_wgheld, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
         r_nfl;
@@ -332,6 +333,8 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         R.bin_count = d.n_bins;
         R.param_begin = par_off;
         R.param_count = d.n_params;
+        R.mat_begin = mat_o;
+        R.mat_count = d.n_materials;
         R.seed = d.seed;
         R.wr = d.wr;
         R.ws = d.ws;
@@ -567,6 +570,8 @@ void Engine::Impl::ensure_bank(bool grad)
         b_cs.alloc(8 * B);
         b_cc.alloc(8 * B);
         b_lp.alloc(8 * B);
+        b_vmat.alloc(4 * B);
+        q_S.alloc(4 * B);
         bank_grad = true;
     }
     ++generation;
@@ -682,6 +687,8 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.cs = grad ? b_cs.as<double>() : nullptr;
     A.cc = grad ? b_cc.as<double>() : nullptr;
     A.lpend = grad ? b_lp.as<double>() : nullptr;
+    A.vmat = grad ? b_vmat.as<uint32_t>() : nullptr;
+    A.qS = grad ? q_S.as<uint32_t>() : nullptr;
     A.qA = q_A.as<uint32_t>();
     A.qC = q_C.as<uint32_t>();
     A.ctr = d_ctr.as<Counters>();
@@ -726,6 +733,11 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     } else {
         if (grad) hipLaunchKernelGGL((k_flight<false, true>), g, b, 0, st, A);
         else hipLaunchKernelGGL((k_flight<false, false>), g, b, 0, st, A);
+    }
+    // gradient runs cross surfaces in a kernel of their own (timed with the flight)
+    if (grad) {
+        if (has_tables) hipLaunchKernelGGL(k_cross<true>, g, b, 0, st, A);
+        else hipLaunchKernelGGL(k_cross<false>, g, b, 0, st, A);
     }
     mark(3, 1);
     mark(4, 0);
@@ -947,6 +959,10 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     if (grad) {
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsum, npp, 0.0);
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsq, npp, 0.0);
+        // every history's closeout clears the S and H entries it used, so the
+        // bank starts each run cleared
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.S, bank_cap * (uint64_t)par_stride, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.H, bank_cap * (uint64_t)par_stride, 0.0);
     }
     HIP_CHECK(hipEventRecord(ev0, stream));
     const bool plain = run && run->mode == MC_RUN_PLAIN;

@@ -240,21 +240,38 @@ __device__ __forceinline__ LrPending lr_pending(const KArgs& A, uint32_t s)
 // Fold one parameter's pending terms into S and H: the scores' coefficients
 // cs, cc of dSigma_s/dtheta, dSigma_c/dtheta, then the increment
 // v = -lp dSigma_t/dtheta + extra of S and its H term v Q.  Terms that are
-// exactly zero are skipped (H never holds -0, so the values are unchanged).
+// exactly zero are skipped (S and H never hold -0, so the values are
+// unchanged).  The updates are device atomics whose result is not used: an
+// entry belongs to one particle, so they apply in program order and leave
+// exactly the sums of load-add-store, without a memory round trip per
+// parameter (h - x and h + (-x) are the same IEEE operation).
 __device__ __forceinline__ void fold_one(const KArgs& A, uint32_t s, int kl, double ds, double dc, double extra,
                                          const LrPending& L)
 {
     // parameter-major rows: lanes of a wave touch neighbouring slots
     const size_t o = (size_t)kl * A.cap + s;
-    double h = 0.0;
-    const bool hc = L.cs != 0.0 || L.cc != 0.0, hq = L.Q != 0.0;
-    if (hc || hq) h = A.H[o];
-    if (hc) h -= L.cs * ds + L.cc * dc;
+    if (L.cs != 0.0 || L.cc != 0.0) atomicAdd(&A.H[o], -(L.cs * ds + L.cc * dc));
     double v = -L.lp * (ds + dc);
     v += extra;
-    A.S[o] += v;
-    if (hq) h += v * L.Q;
-    if (hc || hq) A.H[o] = h;
+    if (v != 0.0) {
+        atomicAdd(&A.S[o], v);
+        if (L.Q != 0.0) atomicAdd(&A.H[o], v * L.Q);
+    }
+}
+
+// S and H entries are read back (at a history's end) in the kernel whose
+// atomics updated them: coherent loads, so no stale first-level cache line
+// is read.
+__device__ __forceinline__ double lr_load(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bit of material m in a slot's visited-materials mask.
+__device__ __forceinline__ uint32_t mat_bit(const DProb& PR, int m)
+{
+    const int l = m - PR.mat_begin;
+    return l < 31 ? 1u << l : 1u << 31;
 }
 
 // Surface crossing: fold the flight that ended on the surface, in material M
@@ -348,34 +365,55 @@ __device__ uint8_t finish_history(const KArgs& A, uint32_t s, int p, const DProb
             T.score = tot;
         }
     }
-    // gradient: a history that touched no bin made no score with v != 0, so
-    // Q, H and the pending coefficients are all zero and it adds nothing
-    if (kGrad && nt != 0) {
-        // scores still pending (surface scores of the cell the particle ends in)
-        double* Srow = A.S + s;
-        double* Hrow = A.H + s;
-        const size_t ks = A.cap;
-        const double Q = A.Qw[s];
-        const double cs = kFlush ? A.cs[s] : 0.0, cc = kFlush ? A.cc[s] : 0.0;
-        if (kFlush && (cs != 0.0 || cc != 0.0)) {
-            const int m = A.pmat[s];
-            if (m >= 0) {
-                const mc_material_desc M = A.mat[m];
-                XsPoint P;
-                const int gm = material_point(A, m, E, P);
-                for (int q = 0; q < M.param_count; ++q) {
-                    const DSig d = param_dsig(A, M, q, E, P, gm);
-                    Hrow[(size_t)(d.k - PR.param_begin) * ks] -= cs * d.ds + cc * d.dc;
+    // gradient: S and H are non-zero only for parameters of the materials
+    // the history has been in.  Their entries are read (when the history
+    // scored: a history that touched no bin made no score with v != 0, so Q,
+    // H and the pending coefficients are zero and it adds nothing) and
+    // cleared for the slot's next history.
+    if (kGrad) {
+        const uint32_t vm = A.vmat[s];
+        if (vm != 0) {
+            double* Srow = A.S + s;
+            double* Hrow = A.H + s;
+            const size_t ks = A.cap;
+            const double Q = A.Qw[s];
+            if (kFlush && nt != 0) {
+                // scores still pending (surface scores of the cell the particle ends in)
+                const double cs = A.cs[s], cc = A.cc[s];
+                const int m = A.pmat[s];
+                if ((cs != 0.0 || cc != 0.0) && m >= 0) {
+                    const mc_material_desc M = A.mat[m];
+                    XsPoint P;
+                    const int gm = material_point(A, m, E, P);
+                    for (int q = 0; q < M.param_count; ++q) {
+                        const DSig d = param_dsig(A, M, q, E, P, gm);
+                        atomicAdd(&Hrow[(size_t)(d.k - PR.param_begin) * ks], -(cs * d.ds + cc * d.dc));
+                    }
                 }
             }
-        }
-        // the history's gradient sum over its scores gv * S = Q S_end - H
-        for (int k = 0; k < PR.param_count; ++k) {
-            const double g = Q * Srow[k * ks] - Hrow[k * ks];
-            if (g != 0.0) {
-                const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
-                atomicAdd(&A.pgsum[j], g);
-                atomicAdd(&A.pgsq[j], g * g);
+            // the history's gradient sum over its scores gv * S = Q S_end - H
+            auto close_param = [&](int k) {
+                double* ps = Srow + (size_t)k * ks;
+                if (nt != 0) {
+                    double* ph = Hrow + (size_t)k * ks;
+                    const double g = Q * lr_load(ps) - lr_load(ph);
+                    if (g != 0.0) {
+                        const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
+                        atomicAdd(&A.pgsum[j], g);
+                        atomicAdd(&A.pgsq[j], g * g);
+                    }
+                    __hip_atomic_store(ph, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __hip_atomic_store(ps, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            };
+            if (vm & (1u << 31)) {
+                for (int k = 0; k < PR.param_count; ++k) close_param(k);
+            } else {
+                for (uint32_t b = vm; b != 0; b &= b - 1) {
+                    const mc_material_desc M = A.mat[PR.mat_begin + __ffs(b) - 1];
+                    for (int q = 0; q < M.param_count; ++q)
+                        close_param(A.mat_param[M.param_begin + q] - PR.param_begin);
+                }
             }
         }
     }
@@ -468,10 +506,9 @@ __device__ __forceinline__ void start_history(const KArgs& A, uint32_t i, uint64
     A.tcount[i] = 0;
     A.status[i] = ST_ALIVE;
     if (kGrad) {
-        for (int k = 0; k < PR.param_count; ++k) {
-            A.S[(size_t)k * A.cap + i] = 0.0;
-            A.H[(size_t)k * A.cap + i] = 0.0;
-        }
+        // S and H of the slot are zero (cleared by the previous history's closeout)
+        const int m0 = A.cell[S.cell].material;
+        A.vmat[i] = m0 >= 0 && A.mat[m0].param_count > 0 ? mat_bit(PR, m0) : 0u;
         A.Qw[i] = 0.0;
         A.cs[i] = 0.0;
         A.cc[i] = 0.0;
@@ -773,7 +810,12 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
     A.rng[s] = rng;
     A.nflight[s] = nfl;
     A.pcell[s] = cell;
-    A.pmat[s] = A.cell[cell].material;
+    const int m1 = A.cell[cell].material;
+    A.pmat[s] = m1;
+    if (kGrad && cell != cell_old && m1 >= 0 && A.mat[m1].param_count > 0) {
+        const uint32_t vm = A.vmat[s], bit = mat_bit(PR, m1);
+        if (!(vm & bit)) A.vmat[s] = vm | bit;
+    }
     if (alive) return ST_ALIVE;
     return finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, nfl);
 }
@@ -897,22 +939,45 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         const uint32_t ci = wave_append(st == ST_COLL, &cn);
         if (st == ST_COLL) A.qC[lo + ci] = s;
         const uint32_t xi = wave_append(st == ST_SURF, &xn);
-        if (st == ST_SURF) xq[xi] = s;
-        __syncthreads();
-        const uint32_t q = xn;          // the same in every thread between the barriers
-        __syncthreads();
-        if (q + blockDim.x > kXq || base + blockDim.x >= hi) {
-            for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
+        if constexpr (kGrad) {
+            // gradient runs: crossings go to this block's segment of queue S
+            // and k_cross takes them (the crossing's folds and closeout would
+            // otherwise raise this kernel's register demand)
+            if (st == ST_SURF) A.qS[lo + xi] = s;
+        } else {
+            if (st == ST_SURF) xq[xi] = s;
             __syncthreads();
-            if (threadIdx.x == 0) { sn += q; xn = 0; }
+            const uint32_t q = xn;          // the same in every thread between the barriers
             __syncthreads();
+            if (q + blockDim.x > kXq || base + blockDim.x >= hi) {
+                for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
+                __syncthreads();
+                if (threadIdx.x == 0) { sn += q; xn = 0; }
+                __syncthreads();
+            }
         }
+    }
+    if constexpr (kGrad) {
+        __syncthreads();
+        if (threadIdx.x == 0) sn = xn;
+        __syncthreads();
     }
     // the collision kernel's workgroup b takes this block's segment
     if (threadIdx.x == 0) {
         A.wg_cnt[2 * blockIdx.x] = sn;
         A.wg_cnt[2 * blockIdx.x + 1] = cn;
     }
+}
+
+// Gradient runs: the surface crossings k_flight queued, workgroup b taking
+// flight workgroup b's segment of queue S.
+template <bool kTables>
+__global__ void __launch_bounds__(1024) k_cross(KArgs A)
+{
+    uint32_t lo, hi;
+    block_range(A.ctr->nA, lo, hi);
+    const uint32_t n = A.wg_cnt[2 * blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) cross_deferred<true>(A, A.qS[lo + i]);
 }
 
 // Event counts of an iteration (profiled runs only): the flight's per-workgroup
@@ -1311,5 +1376,7 @@ template __global__ void k_collision<false, false>(KArgs);
 template __global__ void k_collision<true, false>(KArgs);
 template __global__ void k_collision<false, true>(KArgs);
 template __global__ void k_collision<true, true>(KArgs);
+template __global__ void k_cross<false>(KArgs);
+template __global__ void k_cross<true>(KArgs);
 
 }  // namespace mc
