@@ -170,7 +170,7 @@ struct KArgs {
     const mc_source_desc* src;
     const double* src_cdf;        // per problem: src_count + 1 entries at cdf_off
     const int32_t* cdf_off;
-    const uint64_t* jump;         // 64 LCG jump multipliers
+    const uint64_t* jump;         // LCG jump multipliers, 256 per byte of a history key
     uint64_t jump40;              // g^(2^40): a split copy's stream starts (i + 1) 2^40 draws on
     int32_t split_any;            // some problem can split (importance rises between cells)
     int32_t pad_split;

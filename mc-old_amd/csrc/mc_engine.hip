@@ -469,10 +469,14 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         nbr.insert(nbr.end(), L.begin(), L.end());
         nbr_begin.push_back((int)nbr.size());
     }
-    // LCG jump multipliers g^(stride * 2^k)
-    jump.resize(64);
-    jump[0] = lcg_pow(kRnMult, kRnStride);
-    for (int k = 1; k < 64; ++k) jump[k] = (jump[k - 1] * jump[k - 1]) & kRnMask;
+    // LCG jump multipliers: jump[256 j + v] = g^(stride * v * 256^j), one per
+    // byte value of a history key
+    jump.assign(8 * 256, 1);
+    uint64_t base = lcg_pow(kRnMult, kRnStride);
+    for (int j = 0; j < 8; ++j) {
+        for (int v = 1; v < 256; ++v) jump[256 * j + v] = (jump[256 * j + v - 1] * base) & kRnMask;
+        base = lcg_pow(base, 256);
+    }
 }
 
 void Engine::Impl::upload_static()

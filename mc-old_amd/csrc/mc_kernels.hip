@@ -563,15 +563,50 @@ __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& a_
 }
 
 // Pass 2 of the refill: slot-ordered queue A and new histories, no atomics.
-// Each thread owns a run of consecutive slots of its block's range, so one
-// block scan places every thread's live slots and new histories.  A wave
-// finds the problem and the RN stream multiplier once for its first new key;
-// the other keys of the wave are a short step from it.
+// The block finds the problem and the RN stream multiplier once, for its
+// first new key; every other key of the block is a short step from it.  A
+// block that starts many histories walks its range in tiles of consecutive
+// slots, one slot per lane (the new histories' stores coalesce), ranking each
+// tile with wave ballots and one LDS exchange; one that starts few (the
+// steady state of long histories) gives each thread a run of consecutive
+// slots and ranks them all with a single block scan.
+struct RefillCtx {
+    uint64_t next, lim, chead, hb, jb;
+    int pb;
+    bool cmode, ident;
+};
+
+// Slot i, free and of rank fidx among the free slots, gets a particle: a
+// split copy from the ring, or the history of key next + fidx.
+template <bool kGrad>
+__device__ __forceinline__ void refill_slot(const KArgs& A, const RefillCtx& R, uint32_t i, uint64_t fidx, int& p)
+{
+    if (R.cmode) {
+        start_copy(A, i, (R.chead + fidx) % A.ccap);
+        return;
+    }
+    const uint64_t hid = R.next + fidx;
+    for (int step = 0; p + 1 < A.n_prob && A.run_hb[p + 1] <= hid; ++step) {
+        if (step == 4) { p = find_problem(A, hid); break; }
+        ++p;
+    }
+    start_history<kGrad>(A, i, hid, p, history_seed(R.jb, hid - R.hb, A.jump));
+}
+
+__device__ __forceinline__ bool refill_free(const KArgs& A, uint8_t st, uint32_t i, bool cmode)
+{
+    return st == ST_DEAD || st == ST_EMPTY || (st == ST_HELD && (cmode || A.fcount[i] == 0));
+}
+
 template <bool kGrad>
 __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
 {
     __shared__ uint32_t lds[64];
+    __shared__ uint64_t s_jb;
+    __shared__ int s_pb;
     const uint32_t cap = A.ctr->active_cap;
+    uint32_t lo, hi;
+    block_range(cap, lo, hi);
     const uint64_t next = A.ctr->next_hist, end = A.ctr->end_hist;
     const uint64_t remaining = end > next ? end - next : 0;
     // split copies waiting in the ring go first: a refill that has any places
@@ -592,64 +627,114 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
         free_before += hb;
         free_tot += ht;
     }
+    RefillCtx R;
+    R.next = next;
+    R.chead = chead;
+    R.cmode = cmode;
     // lim: free slots (in rank order) that get a particle
-    const uint64_t lim = cmode ? (pend < free_tot ? pend : (uint64_t)free_tot)
-                               : ((uint64_t)free_tot < remaining ? (uint64_t)free_tot : remaining);
+    R.lim = cmode ? (pend < free_tot ? pend : (uint64_t)free_tot)
+                  : ((uint64_t)free_tot < remaining ? (uint64_t)free_tot : remaining);
     // every free slot gets a history: all slots live, queue A is the identity
-    const bool ident = !cmode && (uint64_t)free_tot <= remaining && live_tot + free_tot == cap;
+    R.ident = !cmode && (uint64_t)free_tot <= remaining && live_tot + free_tot == cap;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A.ctr->claimed = cmode ? 0 : lim;
-        A.ctr->cplaced = cmode ? (uint32_t)lim : 0u;
-        A.ctr->nA = live_tot + (uint32_t)lim;
-        A.ctr->qa_ident = ident ? 1u : 0u;
+        A.ctr->claimed = cmode ? 0 : R.lim;
+        A.ctr->cplaced = cmode ? (uint32_t)R.lim : 0u;
+        A.ctr->nA = live_tot + (uint32_t)R.lim;
+        A.ctr->qa_ident = R.ident ? 1u : 0u;
     }
-    uint32_t lo, hi;
-    block_range(cap, lo, hi);
-    const uint32_t per = (hi - lo + blockDim.x - 1) / blockDim.x;
-    const uint32_t a = min(hi, lo + threadIdx.x * per), b = min(hi, a + per);
-    uint32_t nlive = 0, nfree = 0;
-    for (uint32_t i = a; i < b; ++i) {
-        const uint8_t st = A.status[i];
-        nlive += st == ST_ALIVE;
-        nfree += st == ST_DEAD || st == ST_EMPTY || (st == ST_HELD && (cmode || A.fcount[i] == 0));
-    }
-    uint32_t live_ex, free_ex, lt, ft;
-    block_scan2(nlive, nfree, live_ex, free_ex, lt, ft, lds);
-    uint64_t fidx = (uint64_t)free_before + free_ex;          // rank of my next free slot
-    uint64_t act = (uint64_t)live_before + live_ex + (fidx < lim ? fidx : lim);
-    // the wave's first new key: the smallest fidx of a lane that starts one
-    const bool starts = !cmode && nfree > 0 && fidx < lim;
-    const uint64_t mask = __ballot(starts);
-    uint64_t h0 = 0, j0 = 1;
-    int p0 = 0;
-    if (mask) {
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        h0 = next + (uint64_t)__shfl((unsigned long long)fidx, leader);
-        p0 = find_problem(A, h0);
-        j0 = history_seed(1, h0, A.jump);              // g^(stride * h0)
-    }
-    int p = p0;
-    for (uint32_t i = a; i < b; ++i) {
-        const uint8_t st = A.status[i];
-        if (st == ST_ALIVE) {
-            if (!ident) A.qA[act] = i;
-            ++act;
-        } else if (st == ST_DEAD || st == ST_EMPTY || (st == ST_HELD && (cmode || A.fcount[i] == 0))) {
-            if (fidx < lim) {
-                if (cmode) {
-                    start_copy(A, i, (chead + fidx) % A.ccap);
-                } else {
-                    const uint64_t hid = next + fidx;
-                    while (p + 1 < A.n_prob && A.run_hb[p + 1] <= hid) ++p;
-                    const uint64_t js = history_seed(j0, hid - h0, A.jump);
-                    start_history<kGrad>(A, i, hid, p, js);
-                }
-                if (!ident) A.qA[act] = i;
-                ++act;
-            } else if (st == ST_DEAD) {
-                A.status[i] = ST_EMPTY;
+    R.hb = next + (uint64_t)free_before;                  // the block's first new key
+    // thread 0 finds the block's seed while the statuses load; the first
+    // block scan's barrier publishes it
+    const bool seed = !cmode && (uint64_t)free_before < R.lim;
+    auto find_seed = [&]() {
+        if (threadIdx.x == 0 && seed) {
+            s_pb = find_problem(A, R.hb);
+            s_jb = history_seed(1, R.hb, A.jump);          // g^(stride * hb)
+        }
+    };
+    int p = 0;
+    // particles this block places (its free slots of rank below lim)
+    const uint64_t placed = R.lim > free_before ? R.lim - free_before : 0;
+    if (placed * 16 >= hi - lo) {
+        const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint64_t fbase = free_before, lbase = live_before;
+        // statuses of the next tile are loaded ahead of this tile's scan
+        uint8_t st_next = lo + threadIdx.x < hi ? A.status[lo + threadIdx.x] : ST_SURF;
+        find_seed();
+        bool first = true;
+        for (uint32_t base = lo; base < hi; base += blockDim.x) {
+            const uint32_t i = base + threadIdx.x;
+            const uint8_t st = st_next;                    // ST_SURF: beyond the range
+            st_next = i + blockDim.x < hi ? A.status[i + blockDim.x] : ST_SURF;
+            const bool live = st == ST_ALIVE, fr = st != ST_SURF && refill_free(A, st, i, cmode);
+            const uint64_t ml = __ballot(live), mf = __ballot(fr);
+            if (lane == 0) {
+                lds[2 * wave] = (uint32_t)__popcll(ml);
+                lds[2 * wave + 1] = (uint32_t)__popcll(mf);
             }
-            ++fidx;
+            __syncthreads();
+            uint32_t wl = 0, wf = 0, tl = 0, tf = 0;
+            for (uint32_t k = 0; k < nw; ++k) {
+                const uint32_t x = lds[2 * k], y = lds[2 * k + 1];
+                if (k < wave) { wl += x; wf += y; }
+                tl += x;
+                tf += y;
+            }
+            __syncthreads();
+            if (first) {
+                R.pb = s_pb;
+                R.jb = s_jb;
+                p = R.pb;
+                first = false;
+            }
+            const uint64_t fidx = fbase + wf + (uint64_t)__popcll(mf & below);
+            const uint64_t act = lbase + wl + (uint64_t)__popcll(ml & below) + (fidx < R.lim ? fidx : R.lim);
+            if (live) {
+                if (!R.ident) A.qA[act] = i;
+            } else if (fr) {
+                if (fidx < R.lim) {
+                    refill_slot<kGrad>(A, R, i, fidx, p);
+                    if (!R.ident) A.qA[act] = i;
+                } else if (st == ST_DEAD) {
+                    A.status[i] = ST_EMPTY;
+                }
+            }
+            fbase += tf;
+            lbase += tl;
+        }
+    } else {
+        const uint32_t per = (hi - lo + blockDim.x - 1) / blockDim.x;
+        const uint32_t a = min(hi, lo + threadIdx.x * per), b = min(hi, a + per);
+        uint32_t nlive = 0, nfree = 0;
+        for (uint32_t i = a; i < b; ++i) {
+            const uint8_t st = A.status[i];
+            nlive += st == ST_ALIVE;
+            nfree += refill_free(A, st, i, cmode);
+        }
+        find_seed();
+        uint32_t live_ex, free_ex, lt, ft;
+        block_scan2(nlive, nfree, live_ex, free_ex, lt, ft, lds);
+        R.pb = s_pb;
+        R.jb = s_jb;
+        p = R.pb;
+        uint64_t fidx = (uint64_t)free_before + free_ex;      // rank of my next free slot
+        uint64_t act = (uint64_t)live_before + live_ex + (fidx < R.lim ? fidx : R.lim);
+        for (uint32_t i = a; i < b; ++i) {
+            const uint8_t st = A.status[i];
+            if (st == ST_ALIVE) {
+                if (!R.ident) A.qA[act] = i;
+                ++act;
+            } else if (refill_free(A, st, i, cmode)) {
+                if (fidx < R.lim) {
+                    refill_slot<kGrad>(A, R, i, fidx, p);
+                    if (!R.ident) A.qA[act] = i;
+                    ++act;
+                } else if (st == ST_DEAD) {
+                    A.status[i] = ST_EMPTY;
+                }
+                ++fidx;
+            }
         }
     }
 }

@@ -259,15 +259,17 @@ MC_HD Vec3 scatter_direction(Vec3 d, double mu0, double cos_azi, double sin_azi)
 }
 
 // Per-history stream: RN_init_particle (Random.cpp:196-204) starts history
-// `key` at seed * g^(key * stride).  jump[k] = g^(stride * 2^k) mod 2^63, so
-// the start is one multiply per set bit of key instead of RN_skip_ahead's
-// square-and-multiply.
+// `key` at seed * g^(key * stride).  jump[256 j + v] = g^(stride v 256^j)
+// mod 2^63, so the start is one multiply per byte of key (eight table reads
+// that do not depend on each other) instead of RN_skip_ahead's
+// square-and-multiply; products mod 2^63 do not depend on their order.
 MC_HD uint64_t history_seed(uint64_t seed, uint64_t key, const uint64_t* jump)
 {
-    uint64_t g = 1;
-    for (int k = 0; key; ++k, key >>= 1)
-        if (key & 1) g = (g * jump[k]) & kRnMask;
-    return (g * seed) & kRnMask;
+    uint64_t f[8];
+    for (int j = 0; j < 8; ++j) f[j] = jump[256 * j + ((key >> (8 * j)) & 255u)];
+    uint64_t g = seed & kRnMask;
+    for (int j = 0; j < 8; ++j) g = (g * f[j]) & kRnMask;
+    return g;
 }
 
 }  // namespace mc
