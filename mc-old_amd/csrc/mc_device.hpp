@@ -139,6 +139,7 @@ struct KArgs {
     const double* mat_dens;       // N
     const double* mat_ns;         // per-run scatter density (parameterised materials)
     const double* mat_nc;         // per-run capture density
+    const double* mat_tconst;     // per run: Sigma_t of all-constant materials (NaN otherwise)
     const int32_t* mat_param;     // global parameter index
     const int32_t* mat_grid;      // per material: shared grid of its tables, -1 none, -2 mixed
     const int32_t* me_tab;        // per material entry: table offset, -1 constant nuclide

@@ -121,7 +121,7 @@ struct Engine::Impl {
 
     // device data
     DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
-        d_mat_ns, d_mat_nc, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nfold, d_mfold, d_nuc, d_grid,
+        d_mat_ns, d_mat_nc, d_mat_tconst, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nfold, d_mfold, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
         d_run_hb;
@@ -611,6 +611,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.mat_dens = d_mat_dens.as<double>();
     A.mat_ns = d_mat_ns.as<double>();
     A.mat_nc = d_mat_nc.as<double>();
+    A.mat_tconst = d_mat_tconst.as<double>();
     A.mat_param = d_mat_param.as<int32_t>();
     A.mat_grid = d_mat_grid.as<int32_t>();
     A.me_tab = d_me_tab.as<int32_t>();
@@ -899,6 +900,32 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     upload(d_theta, th, stream);
     upload(d_mat_ns, ns, stream);
     upload(d_mat_nc, nc, stream);
+    // Sigma_t of materials whose nuclides all have constant cross sections,
+    // summed as material_sigt sums it (same terms, same order)
+    std::vector<double> tconst(std::max<size_t>(1, mat.size()), std::nan(""));
+    for (size_t m = 0; m < mat.size(); ++m) {
+        const mc_material_desc& M = mat[m];
+        bool cst = true;
+        for (int k = 0; k < M.nuc_count; ++k) cst = cst && me_tab[M.nuc_begin + k] < 0;
+        if (!cst) continue;
+        if (M.param_count == 0) {
+            double t = 0.0;
+            for (int k = 0; k < M.nuc_count; ++k) {
+                const int j = M.nuc_begin + k;
+                t += me_cst[3 * j + XS_T] * mat_dens[j];
+            }
+            tconst[m] = t;
+        } else {
+            double sv = 0.0, cv = 0.0;
+            for (int k = 0; k < M.nuc_count; ++k) {
+                const int j = M.nuc_begin + k;
+                sv += me_cst[3 * j + XS_S] * ns[j];
+                cv += me_cst[3 * j + XS_C] * nc[j];
+            }
+            tconst[m] = sv + cv;
+        }
+    }
+    upload(d_mat_tconst, tconst, stream);
     // the collision fold's packed parameter table (1/theta as the kernels would
     // divide it: IEEE division on both sides)
     std::vector<int> owner(std::max(1, n_params), 0);

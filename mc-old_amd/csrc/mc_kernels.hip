@@ -931,7 +931,8 @@ __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& 
     const double E = A.E[s];
     double dcol, sigt = 0.0;
     if (C.material >= 0) {
-        sigt = kTables ? A.sigt[s] : material_sigt(A, C.material, E);
+        // without tables every material is all-constant: its per-run Sigma_t
+        sigt = kTables ? A.sigt[s] : A.mat_tconst[C.material];
         dcol = -mc_log(urand(rng)) / sigt;
     } else {
         dcol = kMaxFloatLess;
