@@ -59,6 +59,9 @@ enum { /* surface types, Geometry.h:298-420 */
 };
 enum { MC_BC_VACUUM = -1, MC_BC_TRANSMISSION = 0, MC_BC_REFLECTIVE = 1 };
 enum { MC_NUC_CONSTANT = 1, MC_NUC_TABLE = 2 };
+/* elastic-scatter kinematics of a nuclide of finite mass (deck attribute
+ * kinematics="target_at_rest" (default) or "free_gas")                     */
+enum { MC_KIN_TARGET_AT_REST = 0, MC_KIN_FREE_GAS = 1 };
 enum { MC_DIR_ISOTROPIC = 1, MC_DIR_DELTA = 2 };
 enum { MC_ENERGY_DELTA = 1, MC_ENERGY_UNIFORM = 2 };
 enum { MC_ATTACH_SURFACE = 1, MC_ATTACH_CELL = 2 };
@@ -101,7 +104,8 @@ typedef struct mc_nuclide_desc {
     double A;                       /* target mass ratio; >= MC_MASS_INFINITE: one-speed */
     int32_t kind;                   /* MC_NUC_CONSTANT / MC_NUC_TABLE            */
     int32_t grid_begin, grid_count; /* xs_energy / xs_scatter / xs_capture / xs_total */
-    int32_t pad;
+    int32_t kinematics;             /* elastic scatter: MC_KIN_TARGET_AT_REST or
+                                       MC_KIN_FREE_GAS (Reaction.cpp:27-118, 293.6 K)  */
     double xs_scatter, xs_capture, xs_total;   /* MC_NUC_CONSTANT values (barn) */
 } mc_nuclide_desc;
 

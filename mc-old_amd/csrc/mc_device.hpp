@@ -35,6 +35,7 @@ struct DNuc {            // device nuclide (deduplicated across problems)
     double A;
     double xs_s, xs_c, xs_t;
     int32_t kind, grid, tab, n;   // tab: offset into xs_s/xs_c/xs_t tables
+    int32_t kin, pad;             // elastic-scatter kinematics (MC_KIN_*)
 };
 
 struct DGrid {

@@ -196,6 +196,7 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
             const mc_nuclide_desc& N = d.nuclides[n];
             std::string key((const char*)&N.A, sizeof(double));
             key.append((const char*)&N.kind, sizeof(int32_t));
+            key.append((const char*)&N.kinematics, sizeof(int32_t));
             key.append((const char*)&N.xs_scatter, 3 * sizeof(double));
             if (N.kind == MC_NUC_TABLE) {
                 key.append((const char*)(d.xs_energy + N.grid_begin), sizeof(double) * N.grid_count);
@@ -208,6 +209,7 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
             DNuc D{};
             D.A = N.A;
             D.kind = N.kind;
+            D.kin = N.kinematics;
             D.xs_s = N.xs_scatter;
             D.xs_c = N.xs_capture;
             D.xs_t = N.xs_total;

@@ -1086,7 +1086,28 @@ __global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* 
 // One collision of slot s (general.cpp:121-163): collision tallies, implicit
 // capture, nuclide selection, elastic scatter, weight roulette.  Returns the
 // new status.
-template <bool kTables, bool kGrad>
+// Free-gas scatter of a collision's particle state: inline in the forward
+// collision kernel, out of line where registers are tighter (the drain and
+// the gradient collision kernels).
+struct FreeGasState {
+    double E;
+    Vec3 u;
+    uint64_t rng;
+};
+
+__device__ __forceinline__ FreeGasState free_gas_inline(double A, FreeGasState f)
+{
+    free_gas_scatter(A, f.E, f.u, f.rng);
+    return f;
+}
+
+__device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
+{
+    free_gas_scatter(A, f.E, f.u, f.rng);
+    return f;
+}
+
+template <bool kTables, bool kGrad, bool kInlineFreeGas = true>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
 {
     const int c = A.pcell[s];
@@ -1240,9 +1261,15 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
                 }
             }
         }
-        if (sel >= 0) {
+        const DNuc N = A.nuc[A.mat_nuc[M.nuc_begin + (sel >= 0 ? sel : 0)]];
+        if (sel >= 0 && N.kin == MC_KIN_FREE_GAS && N.A < MC_MASS_INFINITE) {
+            const FreeGasState f = kInlineFreeGas ? free_gas_inline(N.A, FreeGasState{E, u, rng})
+                                                  : free_gas_outline(N.A, FreeGasState{E, u, rng});
+            E = f.E;
+            u = f.u;
+            rng = f.rng;
+        } else if (sel >= 0) {
             // elastic scatter, target at rest (T -> 0 limit of Reaction.cpp:27-118)
-            const DNuc N = A.nuc[A.mat_nuc[M.nuc_begin + sel]];
             const double mu_cm = 2.0 * urand(rng) - 1.0;
             double mu = mu_cm;
             if (N.A < MC_MASS_INFINITE) {
@@ -1285,7 +1312,7 @@ __global__ void __launch_bounds__(1024) k_collision(KArgs A)
     for (; i < n; i += blockDim.x) {
         const uint32_t s = s_next;
         if (i + blockDim.x < n) s_next = q[i + blockDim.x];
-        collision_one<kTables, kGrad>(A, s);
+        collision_one<kTables, kGrad, !kGrad>(A, s);
     }
 }
 
@@ -1320,7 +1347,7 @@ __global__ void __launch_bounds__(1024) k_drain(KArgs A, uint32_t max_events)
             nsurf += crossed;
             if (st == ST_COLL) {
                 ++ncoll;
-                st = collision_one<kTables, kGrad>(A, s);
+                st = collision_one<kTables, kGrad, false>(A, s);
             }
             if (st != ST_ALIVE) break;
         }

@@ -258,6 +258,54 @@ MC_HD Vec3 scatter_direction(Vec3 d, double mu0, double cos_azi, double sin_azi)
     return o;
 }
 
+// ReactionScatter::sample, Reaction.cpp:27-118: elastic scatter of a
+// particle of energy E (eV) and direction u off a free gas of mass ratio A at
+// 293.6 K (constant cross section).  mu0 isotropic in the centre of mass, the
+// target's speed and cosine from the reference's two-branch rejection (its
+// Eq. 37-41), centre-of-mass kinematics, then the lab speed and energy with
+// Particle.cpp:42-56's constants.  The reference leaves the order of the two
+// draws in -log(U) - log(U) cos^2 unspecified (C++ operand order); they are
+// taken left to right.  cos(pi/2 xi) is cos(2 pi (xi / 4)).
+MC_HD void free_gas_scatter(double A, double& E, Vec3& u, uint64_t& rng)
+{
+    const double mu0 = 2.0 * urand(rng) - 1.0;
+    const double beta = sqrt(2.0659834e-11 * A);
+    const double v = 13831.5926439 * sqrt(E) * 100.0;
+    const double y = beta * v;
+    double V, mut;
+    do {
+        double x;
+        if (urand(rng) < 2.0 / (2.0 + 1.7724538509055159 * y)) {
+            const double u1 = urand(rng);
+            const double u2 = urand(rng);
+            x = sqrt(-mc_log(u1 * u2));
+        } else {
+            double c, sn;
+            sincos_2pi(0.25 * urand(rng), c, sn);
+            const double u1 = urand(rng);
+            const double u2 = urand(rng);
+            x = sqrt(-mc_log(u1) - mc_log(u2) * c * c);
+        }
+        V = x / beta;
+        mut = 2.0 * urand(rng) - 1.0;
+    } while (urand(rng) > sqrt(v * v + V * V - 2.0 * v * V * mut) / (v + V));
+    double ca, sa;
+    sincos_2pi(urand(rng), ca, sa);
+    const Vec3 nd = scatter_direction(u, mut, ca, sa);       // the target's direction
+    const Vec3 vl = {v * u.x, v * u.y, v * u.z};
+    const Vec3 cm = {(vl.x + A * (nd.x * V)) / (1.0 + A), (vl.y + A * (nd.y * V)) / (1.0 + A),
+                     (vl.z + A * (nd.z * V)) / (1.0 + A)};  // centre-of-mass velocity
+    Vec3 vc = {vl.x - cm.x, vl.y - cm.y, vl.z - cm.z};
+    const double sc = sqrt(vc.x * vc.x + vc.y * vc.y + vc.z * vc.z);
+    const Vec3 dc = {vc.x / sc, vc.y / sc, vc.z / sc};
+    sincos_2pi(urand(rng), ca, sa);
+    const Vec3 dn = scatter_direction(dc, mu0, ca, sa);
+    const Vec3 w = {sc * dn.x + cm.x, sc * dn.y + cm.y, sc * dn.z + cm.z};
+    const double sp = sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
+    E = 5.2270376e-13 * sp * sp;
+    u = {w.x / sp, w.y / sp, w.z / sp};
+}
+
 // Per-history stream: RN_init_particle (Random.cpp:196-204) starts history
 // `key` at seed * g^(key * stride).  jump[256 j + v] = g^(stride v 256^j)
 // mod 2^63, so the start is one multiply per byte of key (eight table reads

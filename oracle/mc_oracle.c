@@ -663,11 +663,62 @@ static void surface_hit(ostate* st, opart* P, int S)
     cell_importance(st, P);
 }
 
-/* target-at-rest elastic scatter (the T -> 0 limit of Reaction.cpp:27-118):
+/* ReactionScatter::sample, Reaction.cpp:27-118: elastic scatter off a free
+ * gas at 293.6 K with a constant cross section.  mu0 isotropic in the centre
+ * of mass (Distribution.cpp:74-77); target speed and cosine sampled by the
+ * reference's two-branch rejection (its Eq. 37-41); centre-of-mass
+ * kinematics; speed and energy with Particle.cpp:42-56's constants.  The
+ * reference leaves the order of the two draws in -log(U) - log(U) cos^2
+ * unspecified (C++ operand order): they are taken left to right here and on
+ * the GPU.  cos(pi/2 xi) is cos(2 pi (xi/4)).                              */
+void mco_free_gas_scatter(double A, double* E, double u[3], uint64_t* rng)
+{
+    const double mu0 = 2.0 * mco_urand(rng) - 1.0;
+    const double beta = sqrt(2.0659834e-11 * A);
+    const double v = 13831.5926439 * sqrt(*E) * 100.0;
+    const double y = beta * v;
+    double V, mut;
+    do {
+        double x;
+        if (mco_urand(rng) < 2.0 / (2.0 + 1.7724538509055159 * y)) {
+            const double u1 = mco_urand(rng);
+            const double u2 = mco_urand(rng);
+            x = sqrt(-mco_log(u1 * u2));
+        } else {
+            double c, sn;
+            mco_sincos_2pi(0.25 * mco_urand(rng), &c, &sn);
+            const double u1 = mco_urand(rng);
+            const double u2 = mco_urand(rng);
+            x = sqrt(-mco_log(u1) - mco_log(u2) * c * c);
+        }
+        V = x / beta;
+        mut = 2.0 * mco_urand(rng) - 1.0;
+    } while (mco_urand(rng) > sqrt(v * v + V * V - 2.0 * v * V * mut) / (v + V));
+    double nd[3], cm[3], vc[3], dc[3], dn[3], vl[3];
+    mco_scatter_direction(u, mut, rng, nd);                   /* target direction */
+    for (int k = 0; k < 3; ++k) {
+        vl[k] = v * u[k];
+        cm[k] = (vl[k] + A * (nd[k] * V)) / (1.0 + A);         /* centre-of-mass velocity */
+        vc[k] = vl[k] - cm[k];
+    }
+    const double sc = sqrt(vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2]);
+    for (int k = 0; k < 3; ++k) dc[k] = vc[k] / sc;
+    mco_scatter_direction(dc, mu0, rng, dn);
+    for (int k = 0; k < 3; ++k) vl[k] = sc * dn[k] + cm[k];
+    const double sp = sqrt(vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]);
+    *E = 5.2270376e-13 * sp * sp;
+    for (int k = 0; k < 3; ++k) u[k] = vl[k] / sp;
+}
+
+/* elastic scatter: free gas (above) or target at rest, its T -> 0 limit:
  * mu_cm isotropic (Distribution.cpp:74-77), then scatter_direction          */
 static void scatter(ostate* st, opart* P, int nuc)
 {
     const mc_nuclide_desc* N = &st->d->nuclides[nuc];
+    if (N->A < MC_MASS_INFINITE && N->kinematics == MC_KIN_FREE_GAS) {
+        mco_free_gas_scatter(N->A, &P->E, P->u, &P->rng);
+        return;
+    }
     const double mu_cm = 2.0 * mco_urand(&P->rng) - 1.0;
     double mu = mu_cm;
     if (N->A < MC_MASS_INFINITE) {

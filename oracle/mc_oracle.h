@@ -35,6 +35,10 @@ void   mco_scatter_direction(const double dir[3], double mu0, uint64_t* rng, dou
 double mco_log(double x);
 void   mco_sincos_2pi(double xi, double* c, double* s);
 
+/* --- Reaction.cpp:27-118 ------------------------------------------------------ */
+/* free-gas elastic scatter at 293.6 K: updates *E (eV) and the direction u */
+void   mco_free_gas_scatter(double A, double* E, double u[3], uint64_t* rng);
+
 /* --- Geometry.cpp ------------------------------------------------------------ */
 double mco_surface_eval(const mc_surface_desc* S, const double p[3]);
 double mco_surface_distance(const mc_surface_desc* S, const double p[3], const double u[3]);

@@ -170,3 +170,63 @@ def importance_slab(L: float, c: float, nb: int, n: int, ratio: float) -> str:
         deck = deck.replace(f'<cell name="bin{k}" material="slab">',
                             f'<cell name="bin{k}" material="slab" importance="{ratio ** k!r}">')
     return deck
+
+
+def free_gas_slab(n: int, L: float = 5.0, E0: float = 1.0, kinematics: str = "free_gas") -> str:
+    """A hydrogen-carbon slab (constant cross sections, A = 1 and 12) hit by a
+    normally incident E0 eV beam: collisions thermalize the particles, so
+    free-gas kinematics decides their energies.  Transmission, reflection
+    and the reflected energy spectrum."""
+    return f"""<?xml version='1.0' encoding='UTF-8'?>
+<simulation>
+    <description name="free-gas slab" samples="{n}" seed="1"/>
+</simulation>
+<nuclides>
+    <nuclide name="h" A="1.0" kinematics="{kinematics}">
+        <scatter xs="20.0"/>
+        <capture xs="0.3"/>
+    </nuclide>
+    <nuclide name="c" A="12.0" kinematics="{kinematics}">
+        <scatter xs="4.7"/>
+        <capture xs="0.0035"/>
+    </nuclide>
+</nuclides>
+<materials>
+    <material name="hc">
+        <nuclide name="h" density="0.06"/>
+        <nuclide name="c" density="0.03"/>
+    </material>
+</materials>
+<surfaces>
+    <plane_x name="px0" x="0.0"/>
+    <plane_x name="px1" x="{L!r}"/>
+</surfaces>
+<cells>
+    <cell name="slab" material="hc">
+        <surface name="px0" sense="+1"/>
+        <surface name="px1" sense="-1"/>
+    </cell>
+    <cell name="left outside" importance="0.0">
+        <surface name="px0" sense="-1"/>
+    </cell>
+    <cell name="right outside" importance="0.0">
+        <surface name="px1" sense="+1"/>
+    </cell>
+</cells>
+<distributions>
+    <delta name="beam" datatype="point" x="1.0" y="0.0" z="0.0"/>
+    <delta name="e0" datatype="double" val="{E0!r}"/>
+</distributions>
+<sources>
+    <point x="1e-9" y="0.0" z="0.0" direction="beam" energy="e0"/>
+</sources>
+<estimators>
+    <estimator name="transmission" scores="cross">
+        <surface name="px1"/>
+    </estimator>
+    <estimator name="reflection" scores="cross">
+        <surface name="px0"/>
+        <filter type="energy" grid_lethargy="1e-5 10.0 40"/>
+    </estimator>
+</estimators>
+"""

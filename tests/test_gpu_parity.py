@@ -120,6 +120,22 @@ def test_batched_equals_plain_equals_oracle():
         off += int(N[i])
 
 
+# drain_divisor 1: the run's tail goes to the drain kernel; -1: event iterations only
+@pytest.mark.parametrize("drain", [1, -1])
+def test_free_gas_histories_bitexact(drain):
+    """Free-gas scattering (Reaction.cpp:27-118): a 1 eV beam thermalizing in
+    a hydrogen-carbon slab, every history bit-exact against the oracle."""
+    from mc_testlib import free_gas_slab
+    n = 5000
+    p = mc.Problem.from_string(free_gas_slab(n))
+    g = mc.Engine(p, bank_capacity=1 << 12, drain_divisor=drain).forward(trace=n)
+    o = oracle_run(p, n, trace=n)
+    assert_traces_equal(g.trace, o.trace, "free gas")
+    assert g.flights == o.flights
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(g.sq, o.sq, rtol=1e-12, atol=1e-300)
+
+
 def test_error_paths_on_device():
     # a copy ring that overflows (importance x8 per bin, 64 bank slots) ends the
     # run with a capacity status rather than losing copies silently
