@@ -60,7 +60,8 @@ enum { /* surface types, Geometry.h:298-420 */
 enum { MC_BC_VACUUM = -1, MC_BC_TRANSMISSION = 0, MC_BC_REFLECTIVE = 1 };
 enum { MC_NUC_CONSTANT = 1, MC_NUC_TABLE = 2 };
 /* elastic-scatter kinematics of a nuclide of finite mass (deck attribute
- * kinematics="target_at_rest" (default) or "free_gas")                     */
+ * kinematics="free_gas" (default: the reference's, Reaction.cpp:27-118) or
+ * "target_at_rest", its T -> 0 limit)                                       */
 enum { MC_KIN_TARGET_AT_REST = 0, MC_KIN_FREE_GAS = 1 };
 enum { MC_DIR_ISOTROPIC = 1, MC_DIR_DELTA = 2 };
 enum { MC_ENERGY_DELTA = 1, MC_ENERGY_UNIFORM = 2 };

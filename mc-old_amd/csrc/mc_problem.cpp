@@ -223,12 +223,14 @@ void load_problem_xml(Problem& P, const std::string& xml, const std::string& xs_
             }
             need(N.A > 0.0, MC_ERR_INPUT, "nuclide " + name + " has non-positive mass");
             // the reference scatters every nuclide off a free gas at 293.6 K
-            // (Reaction.cpp:27-118); target at rest is its T -> 0 limit
-            N.kinematics = MC_KIN_TARGET_AT_REST;
+            // (Reaction.cpp:27-118); a deck may ask for the target-at-rest
+            // limit (T -> 0) instead
+            N.kinematics = MC_KIN_FREE_GAS;
             if (n->has("kinematics")) {
                 const std::string k = n->value("kinematics");
-                if (k == "free_gas") N.kinematics = MC_KIN_FREE_GAS;
-                else need(k == "target_at_rest", MC_ERR_INPUT, "unknown kinematics '" + k + "' of nuclide " + name);
+                need(k == "free_gas" || k == "target_at_rest", MC_ERR_INPUT,
+                     "unknown kinematics '" + k + "' of nuclide " + name);
+                N.kinematics = k == "free_gas" ? MC_KIN_FREE_GAS : MC_KIN_TARGET_AT_REST;
             }
             P.nuclides.push_back(N);
             P.nuclide_names.push_back(name);

@@ -157,7 +157,9 @@ def write_shield_nuclides(xs_dir: str, seed: int = 7) -> None:
 
 
 def _shield_common(samples: float, seed: int) -> tuple[str, str]:
-    nucs = "\n".join(f'    <nuclide name="{n}" ZAID="{ZAIDS[n]}"/>' for n, *_ in SHIELD_NUCLIDES)
+    # elastic scattering with the target at rest (the workloads' kinematics)
+    nucs = "\n".join(f'    <nuclide name="{n}" ZAID="{ZAIDS[n]}" kinematics="target_at_rest"/>'
+                     for n, *_ in SHIELD_NUCLIDES)
     mats = []
     for m, comp in SHIELD_MATERIALS.items():
         inner = "\n".join(f'        <nuclide name="{n}" density="{_f(d)}"/>' for n, d in comp.items())
