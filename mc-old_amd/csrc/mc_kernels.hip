@@ -1,54 +1,39 @@
 // mc_kernels.hip — event-based transport kernels for gfx950 (wave64).
 //
 // One event iteration over the structure-of-arrays particle bank:
-//   k_refill_count    per-workgroup counts of live and free slots (and the
-//                     counter roll-over from the previous iteration)
-//   k_refill_scatter  prefix over the counts, then queue A = live slots plus
-//                     new histories started in free slots, in slot order
-//                     (Simulator::start's sample loop, handler.cpp:19-36)
+//   k_refill_count    per-workgroup counts of live, free and family-held slots
+//                     (and the counter roll-over from the previous iteration)
+//   k_refill_scatter  prefix over the counts; split copies waiting in the
+//                     copy ring, else new histories, go into free slots; queue
+//                     A = live slots plus the new ones, in slot order, not
+//                     written while every slot is live (Simulator::start's
+//                     sample loop, handler.cpp:19-36)
 //   k_xs_lookup       Sigma_t(E) for every particle in A (Material::SigmaT,
 //                     Material.cpp:50-57) — continuous-energy problems only
 //   k_flight          distance to collision vs nearest surface, move, track-
 //                     length tallies (random_walk + move_particle, general.cpp:
-//                     40-83, 177-211); a flight that ends on a surface crosses
-//                     it here: boundary conditions, cell search, surface tallies,
-//                     importance, weight roulette (general.cpp:89-115,
+//                     40-83, 177-211); flights that end on a surface are crossed
+//                     by the workgroup (gradient runs: by k_cross): boundary
+//                     conditions, cell search, surface tallies, importance
+//                     split or roulette, weight roulette (general.cpp:89-115,
 //                     population_control.cpp:9-49); collisions go to the
 //                     workgroup's own segment of queue C
 //   k_collision       per workgroup, the segment its flight workgroup filled:
 //                     collision tallies, implicit capture, nuclide selection,
 //                     elastic scatter, weight roulette (general.cpp:121-163)
-// Queue A is built without atomics: wave ballots rank lanes inside a block,
-// blocks rank among themselves from per-workgroup counts, so it keeps slot
-// order and consecutive lanes touch consecutive bank entries.  Queue C is
-// block-local: a wave ballot ranks its lanes and one LDS atomic per wave
-// places the wave, so lanes stay in slot order within a wave.
-// A history that ends is closed out where it ends: the bins it touched are
-// added to the FP64 moment arrays (Estimator::end_history, Estimator.cpp:339-346).
+// k_drain carries the last particles of a run from event to event through the
+// same per-particle routines.  Queue A is built without atomics: workgroups
+// rank themselves from per-workgroup counts and rank their slots with block
+// scans, so it keeps slot order and consecutive lanes touch consecutive bank
+// entries.  Queue C is block-local: a wave ballot ranks its lanes and one LDS
+// atomic per wave places the wave.  A history that ends is closed out where it
+// ends: the bins it touched are added to the FP64 moment arrays
+// (Estimator::end_history, Estimator.cpp:339-346).
 #include "mc_device.hpp"
 
 namespace mc {
 
 // ------------------------------------------------------------ block helpers
-
-// Rank of a set flag among the block's set flags (in thread order) and the
-// block total.  lds: at least 16 words.
-__device__ __forceinline__ uint32_t block_flag_rank(bool f, uint32_t& total, uint32_t* lds)
-{
-    const uint64_t m = __ballot(f);
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (lane == 0) lds[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t before = 0, tot = 0;
-    for (uint32_t k = 0; k < nw; ++k) {
-        const uint32_t c = lds[k];
-        before += k < wave ? c : 0u;
-        tot += c;
-    }
-    __syncthreads();
-    total = tot;
-    return before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
 
 // Sum over the block of two counters.  lds: at least 32 words.
 __device__ __forceinline__ void block_sum2(uint32_t& a, uint32_t& b, uint32_t* lds)
