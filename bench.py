@@ -15,6 +15,10 @@ Also reported on the same JSON line:
                     event, device time from per-kernel HIP events)
   tally_check       GPU tallies vs the CPU oracle sample (combined-sigma z scores)
   cpu_baseline      the oracle (C restatement, 1 core) on a bounded sample
+  cpu_baseline_parallel
+                    the same oracle as independent processes over disjoint
+                    history ranges, one per host core of this process (up to
+                    16), started before the GPU is used; throughput summed
 """
 from __future__ import annotations
 
@@ -70,6 +74,46 @@ def cpu_baseline(mc, decks, xs, n_sample):
     r = oracle_run(p, n_sample, key_offset=10 ** 12)
     dt = time.perf_counter() - t
     return r, dt
+
+
+def cpu_worker(offset, n, xs):
+    """One process of the parallel CPU baseline: n oracle histories of the
+    shield from key offset on (no GPU use)."""
+    from mc_testlib import oracle_run
+    mc = importlib.import_module("mc-old_amd")
+    decks = importlib.import_module("mc-old_amd.decks")
+    p = mc.Problem.from_string(decks.shield_deck(n), xs)
+    t = time.perf_counter()
+    oracle_run(p, n, key_offset=offset)
+    print(json.dumps({"seconds": time.perf_counter() - t, "histories": n}), flush=True)
+
+
+def cpu_parallel(xs, n_each):
+    """The oracle as P independent processes on disjoint key ranges (P = host
+    cores available to this process, at most 16), wall time around all of
+    them; run before this process touches the GPU."""
+    import subprocess
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    P = max(1, min(16, cores))
+    t = time.perf_counter()
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                               str(2 * 10 ** 12 + k * n_each), str(n_each), xs],
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, cwd=ROOT)
+             for k in range(P)]
+    outs = [pr.communicate(timeout=600) for pr in procs]
+    wall = time.perf_counter() - t
+    if any(pr.returncode != 0 for pr in procs):
+        raise RuntimeError("cpu worker failed: " + " | ".join(o[1][-300:] for o in outs))
+    # each process times its own histories (interpreter start-up excluded);
+    # they run side by side, so the slowest one bounds the aggregate
+    secs = [json.loads([l for l in o[0].splitlines() if l.startswith("{")][-1])["seconds"] for o in outs]
+    return {"value": P * n_each / max(secs), "unit": "histories/s", "cores": P, "kind": "port",
+            "sample": f"{P} processes x {n_each} histories of the same shield deck on disjoint keys "
+                      f"(oracle C restatement), slowest {max(secs):.1f} s ({wall:.1f} s wall with start-up), "
+                      f"{platform.processor() or platform.machine()}, nproc={os.cpu_count()}"}
 
 
 def zscores(gm, gu, om, ou, min_rel=0.2):
@@ -157,7 +201,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--xs-only", action="store_true", help="only the standalone cross-section lookup")
+    ap.add_argument("--cpu-worker", nargs=3, metavar=("OFFSET", "N", "XS"), help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(int(args.cpu_worker[0]), int(args.cpu_worker[1]), args.cpu_worker[2])
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
               "(histories shard with no communication; see the sharding design)", file=sys.stderr)
@@ -178,6 +226,15 @@ def main():
             log(f"traffic: {json.dumps(traffic)}")
         except Exception as ex:      # the counters are a report, not a gate
             log(f"traffic passes skipped: {ex}")
+
+    cpu_par = None
+    if not args.quick:
+        # child processes too, so also before this process uses the GPU
+        try:
+            cpu_par = cpu_parallel(shield_xs_dir(), max(1000, args.cpu_sample // 2))
+            log(f"cpu oracle, {cpu_par['cores']} processes: {cpu_par['value']:.4g} hist/s")
+        except Exception as ex:      # a reported baseline, not a gate
+            log(f"parallel cpu baseline skipped: {ex}")
 
     mc = importlib.import_module("mc-old_amd")
     decks = importlib.import_module("mc-old_amd.decks")
@@ -233,6 +290,8 @@ def main():
                            "sample": f"{n_cpu} histories of the same shield deck (oracle C restatement, "
                                      f"{dt_cpu:.1f} s, {platform.processor() or platform.machine()}, "
                                      f"nproc={os.cpu_count()})"}
+    if cpu_par:
+        out["cpu_baseline_parallel"] = cpu_par
     log(f"cpu oracle: {n_cpu / dt_cpu:.4g} hist/s; tally z max {out['tally_check']['max_abs_z']:.2f}")
 
     if not args.quick:
