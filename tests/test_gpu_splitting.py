@@ -83,3 +83,43 @@ def test_split_in_gradient_run_is_rejected():
     with pytest.raises(mc.MCError) as e:
         mc.Engine(p).backward(np.ones(p.n_bins))
     assert e.value.status == mc.MC_ERR_UNSUPPORTED
+
+
+def test_split_with_collision_and_track_length_estimators():
+    """Splitting in a slab scored by track-length and collision estimators of
+    every reaction (bins 2-11) besides T and R: members of a family add into
+    one row, closed out once."""
+    from mc_testlib import REACTION_SCORES
+    n = 10000
+    sc = " ".join(REACTION_SCORES)
+    deck = importance_slab(3.0, 0.8, 6, n, 2.0)
+    extra = "".join(f'    <estimator name="{t}" type="{t}" scores="{sc}">\n' +
+                    "".join(f'        <cell name="bin{k}"/>\n' for k in range(6)) + "    </estimator>\n"
+                    for t in ("TL", "C"))
+    deck = deck.replace("</estimators>", extra + "</estimators>")
+    p = mc.Problem.from_string(deck)
+    _check(p, n, bank_capacity=1 << 13)
+
+
+def test_split_with_free_gas_and_energy_tally():
+    """Importance rising through a thermalizing hydrogen-carbon slab
+    (free-gas scattering, energy-filtered reflection)."""
+    from mc_testlib import free_gas_slab
+    n = 3000
+    # the slab as two cells of importance 1 and 2
+    deck = free_gas_slab(n)
+    deck = deck.replace('<plane_x name="px1" x="5.0"/>', '<plane_x name="pxm" x="2.5"/>\n    <plane_x name="px1" x="5.0"/>')
+    deck = deck.replace("""    <cell name="slab" material="hc">
+        <surface name="px0" sense="+1"/>
+        <surface name="px1" sense="-1"/>
+    </cell>""", """    <cell name="front" material="hc">
+        <surface name="px0" sense="+1"/>
+        <surface name="pxm" sense="-1"/>
+    </cell>
+    <cell name="back" material="hc" importance="2.0">
+        <surface name="pxm" sense="+1"/>
+        <surface name="px1" sense="-1"/>
+    </cell>""")
+    p = mc.Problem.from_string(deck)
+    assert p.n_bins > 2
+    _check(p, n, bank_capacity=1 << 12)
