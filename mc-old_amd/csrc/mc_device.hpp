@@ -204,6 +204,12 @@ struct KArgs {
     int32_t *ccell, *cprob, *cfam;
     uint32_t* cnfl;
     uint64_t ccap;
+    // gradient runs that split: a copy's score function S (parameter-major,
+    // row stride ccap) and visited-materials mask, taken from the particle at
+    // the split; per family row slot the family's gradient sum (row stride cap)
+    double* rS;
+    uint32_t* rvmat;
+    double* famG;
     double* hist;
     // likelihood-ratio state: score function S and H = sum of dS * Q, parameter-
     // major [param][slot] with row stride cap,

@@ -129,7 +129,7 @@ struct Engine::Impl {
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
         b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, b_vmat, q_S, q_A, q_C, d_ctr, d_wgcnt, b_mat;
     // split families and the copy ring (problems that can split only)
-    DevBuf b_fam, b_fcount, d_wgheld, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
+    DevBuf b_fam, b_fcount, d_wgheld, r_S, r_vmat, b_famG, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
         r_nfl;
     uint64_t ring_cap = 0;
     uint32_t h_active_cap = 0;
@@ -578,6 +578,12 @@ void Engine::Impl::ensure_bank(bool grad)
         b_lp.alloc(8 * B);
         b_vmat.alloc(4 * B);
         q_S.alloc(4 * B);
+        if (split_any) {
+            // split copies carry S; families sum their members' gradients
+            r_S.alloc(8 * ring_cap * (uint64_t)par_stride);
+            r_vmat.alloc(4 * ring_cap);
+            b_famG.alloc(8 * B * (uint64_t)par_stride);
+        }
         bank_grad = true;
     }
     ++generation;
@@ -695,6 +701,9 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.cc = grad ? b_cc.as<double>() : nullptr;
     A.lpend = grad ? b_lp.as<double>() : nullptr;
     A.vmat = grad ? b_vmat.as<uint32_t>() : nullptr;
+    A.rS = grad && split_any ? r_S.as<double>() : nullptr;
+    A.rvmat = grad && split_any ? r_vmat.as<uint32_t>() : nullptr;
+    A.famG = grad && split_any ? b_famG.as<double>() : nullptr;
     A.qS = grad ? q_S.as<uint32_t>() : nullptr;
     A.qA = q_A.as<uint32_t>();
     A.qC = q_C.as<uint32_t>();
@@ -996,6 +1005,8 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         // bank starts each run cleared
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.S, bank_cap * (uint64_t)par_stride, 0.0);
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.H, bank_cap * (uint64_t)par_stride, 0.0);
+        if (split_any)
+            hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.famG, bank_cap * (uint64_t)par_stride, 0.0);
     }
     HIP_CHECK(hipEventRecord(ev0, stream));
     const bool plain = run && run->mode == MC_RUN_PLAIN;
@@ -1026,7 +1037,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
     if (err_lost) throw Error(MC_ERR_LOST_PARTICLE, std::to_string(err_lost) + " particle(s) lost");
     if (err_split)
-        throw Error(MC_ERR_UNSUPPORTED, "importance splitting (ratio > 1) is not supported in gradient runs");
+        throw Error(MC_ERR_ARGUMENT, "a particle split in a problem without an importance rise");
     if (err_ring)
         throw Error(MC_ERR_CAPACITY, "split copies overflowed the copy ring (" + std::to_string(ring_cap) +
                                          " entries, 8 per bank slot); raise bank_capacity");

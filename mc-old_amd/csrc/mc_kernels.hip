@@ -288,9 +288,74 @@ __device__ __forceinline__ void close_bin(const KArgs& A, double* row, const DPr
     }
 }
 
+// Gradient closeout of particle s (Estimator::end_history for the score
+// function): S and H are non-zero only for parameters of the materials the
+// particle has been in.  Their entries are read (when it scored: a history
+// that touched no bin made no score with v != 0, so Q, H and the pending
+// coefficients are zero and it adds nothing) and cleared for the slot's next
+// particle.  Its sum Q S_end - H goes to the workgroup's moments, or for a
+// member of a split family (fam >= 0) to the family's sum, which the family's
+// last member closes out.
+template <bool kFlush>
+__device__ __forceinline__ void grad_closeout(const KArgs& A, uint32_t s, const DProb& PR, double E, bool scored, int fam)
+{
+    {
+        const uint32_t vm = A.vmat[s];
+        if (vm != 0) {
+            double* Srow = A.S + s;
+            double* Hrow = A.H + s;
+            const size_t ks = A.cap;
+            const double Q = A.Qw[s];
+            if (kFlush && scored) {
+                // scores still pending (surface scores of the cell the particle ends in)
+                const double cs = A.cs[s], cc = A.cc[s];
+                const int m = A.pmat[s];
+                if ((cs != 0.0 || cc != 0.0) && m >= 0) {
+                    const mc_material_desc M = A.mat[m];
+                    XsPoint P;
+                    const int gm = material_point(A, m, E, P);
+                    for (int q = 0; q < M.param_count; ++q) {
+                        const DSig d = param_dsig(A, M, q, E, P, gm);
+                        atomicAdd(&Hrow[(size_t)(d.k - PR.param_begin) * ks], -(cs * d.ds + cc * d.dc));
+                    }
+                }
+            }
+            // the history's gradient sum over its scores gv * S = Q S_end - H
+            auto close_param = [&](int k) {
+                double* ps = Srow + (size_t)k * ks;
+                if (scored) {
+                    double* ph = Hrow + (size_t)k * ks;
+                    const double g = Q * lr_load(ps) - lr_load(ph);
+                    if (g != 0.0) {
+                        if (fam >= 0) {
+                            atomicAdd(&A.famG[(size_t)k * ks + fam], g);     // the family's sum
+                        } else {
+                            const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
+                            atomicAdd(&A.pgsum[j], g);
+                            atomicAdd(&A.pgsq[j], g * g);
+                        }
+                    }
+                    __hip_atomic_store(ph, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                __hip_atomic_store(ps, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            };
+            if (vm & (1u << 31)) {
+                for (int k = 0; k < PR.param_count; ++k) close_param(k);
+            } else {
+                for (uint32_t b = vm; b != 0; b &= b - 1) {
+                    const mc_material_desc M = A.mat[PR.mat_begin + __ffs(b) - 1];
+                    for (int q = 0; q < M.param_count; ++q)
+                        close_param(A.mat_param[M.param_begin + q] - PR.param_begin);
+                }
+            }
+        }
+    }
+}
+
 // A member of a split family ends: the last one closes out the family's row
 // (Estimator::end_history for the whole history, parent and copies).  The
 // slot's status becomes ST_HELD: its row may still be a family's.
+template <bool kGrad>
 __device__ uint8_t finish_family(const KArgs& A, uint32_t s, const DProb& PR)
 {
     const int f = A.fam[s];
@@ -301,6 +366,19 @@ __device__ uint8_t finish_family(const KArgs& A, uint32_t s, const DProb& PR)
         double* row = A.hist + (size_t)f * (size_t)A.bin_stride;
         for (int b = 0; b < PR.bin_count; ++b) close_bin(A, row, PR, b);
         A.tcount[f] = 0;
+        if (kGrad) {
+            // the history's gradient: the sum over its members
+            for (int k = 0; k < PR.param_count; ++k) {
+                double* pg = A.famG + (size_t)k * A.cap + f;
+                const double G = lr_load(pg);
+                if (G != 0.0) {
+                    const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
+                    atomicAdd(&A.pgsum[j], G);
+                    atomicAdd(&A.pgsq[j], G * G);
+                    __hip_atomic_store(pg, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
     }
     return ST_HELD;
 }
@@ -312,7 +390,10 @@ template <bool kGrad, bool kFlush>
 __device__ uint8_t finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
                                   double E, uint32_t nfl)
 {
-    if (A.split_any && A.fam[s] >= 0) return finish_family(A, s, PR);
+    if (A.split_any && A.fam[s] >= 0) {
+        if (kGrad) grad_closeout<kFlush>(A, s, PR, E, true, A.fam[s]);
+        return finish_family<kGrad>(A, s, PR);
+    }
     double* row = A.hist + (size_t)s * (size_t)A.bin_stride;
     const uint32_t nt = A.tcount[s];
     const uint32_t* tb = A.tbins + (size_t)s * kTouched;
@@ -350,58 +431,7 @@ __device__ uint8_t finish_history(const KArgs& A, uint32_t s, int p, const DProb
             T.score = tot;
         }
     }
-    // gradient: S and H are non-zero only for parameters of the materials
-    // the history has been in.  Their entries are read (when the history
-    // scored: a history that touched no bin made no score with v != 0, so Q,
-    // H and the pending coefficients are zero and it adds nothing) and
-    // cleared for the slot's next history.
-    if (kGrad) {
-        const uint32_t vm = A.vmat[s];
-        if (vm != 0) {
-            double* Srow = A.S + s;
-            double* Hrow = A.H + s;
-            const size_t ks = A.cap;
-            const double Q = A.Qw[s];
-            if (kFlush && nt != 0) {
-                // scores still pending (surface scores of the cell the particle ends in)
-                const double cs = A.cs[s], cc = A.cc[s];
-                const int m = A.pmat[s];
-                if ((cs != 0.0 || cc != 0.0) && m >= 0) {
-                    const mc_material_desc M = A.mat[m];
-                    XsPoint P;
-                    const int gm = material_point(A, m, E, P);
-                    for (int q = 0; q < M.param_count; ++q) {
-                        const DSig d = param_dsig(A, M, q, E, P, gm);
-                        atomicAdd(&Hrow[(size_t)(d.k - PR.param_begin) * ks], -(cs * d.ds + cc * d.dc));
-                    }
-                }
-            }
-            // the history's gradient sum over its scores gv * S = Q S_end - H
-            auto close_param = [&](int k) {
-                double* ps = Srow + (size_t)k * ks;
-                if (nt != 0) {
-                    double* ph = Hrow + (size_t)k * ks;
-                    const double g = Q * lr_load(ps) - lr_load(ph);
-                    if (g != 0.0) {
-                        const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
-                        atomicAdd(&A.pgsum[j], g);
-                        atomicAdd(&A.pgsq[j], g * g);
-                    }
-                    __hip_atomic_store(ph, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                __hip_atomic_store(ps, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            };
-            if (vm & (1u << 31)) {
-                for (int k = 0; k < PR.param_count; ++k) close_param(k);
-            } else {
-                for (uint32_t b = vm; b != 0; b &= b - 1) {
-                    const mc_material_desc M = A.mat[PR.mat_begin + __ffs(b) - 1];
-                    for (int q = 0; q < M.param_count; ++q)
-                        close_param(A.mat_param[M.param_begin + q] - PR.param_begin);
-                }
-            }
-        }
-    }
+    if (kGrad) grad_closeout<kFlush>(A, s, PR, E, nt != 0, -1);
     return ST_DEAD;
 }
 
@@ -503,9 +533,22 @@ __device__ __forceinline__ void start_history(const KArgs& A, uint32_t i, uint64
 
 // A split copy from ring entry k takes slot i (its family's row stays where
 // the family began; the slot's own row is not touched).
+template <bool kGrad>
 __device__ __forceinline__ void start_copy(const KArgs& A, uint32_t i, uint64_t k)
 {
     const int c = A.ccell[k];
+    if (kGrad) {
+        // S of the particle at the split; the slot's S and H are clear (its
+        // last history's closeout cleared them), and so are Q and the rest
+        const int p = A.cprob[k];
+        const DProb PR = A.prob[p];
+        A.vmat[i] = A.rvmat[k];
+        for (int q = 0; q < PR.param_count; ++q) A.S[(size_t)q * A.cap + i] = A.rS[(size_t)q * A.ccap + k];
+        A.Qw[i] = 0.0;
+        A.cs[i] = 0.0;
+        A.cc[i] = 0.0;
+        A.lpend[i] = 0.0;
+    }
     A.x[i] = A.cx[k]; A.y[i] = A.cy[k]; A.z[i] = A.cz[k];
     A.u[i] = A.cu[k]; A.v[i] = A.cv[k]; A.w[i] = A.cw[k];
     A.E[i] = A.cE[k];
@@ -567,7 +610,7 @@ template <bool kGrad>
 __device__ __forceinline__ void refill_slot(const KArgs& A, const RefillCtx& R, uint32_t i, uint64_t fidx, int& p)
 {
     if (R.cmode) {
-        start_copy(A, i, (R.chead + fidx) % A.ccap);
+        start_copy<kGrad>(A, i, (R.chead + fidx) % A.ccap);
         return;
     }
     const uint64_t hid = R.next + fidx;
@@ -779,7 +822,7 @@ template <bool kGrad>
 __device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, Vec3 p, Vec3 u, double E, double w,
                                           uint64_t rng, uint32_t nfl, int cell, int p_id)
 {
-    if (kGrad || !A.split_any) {
+    if (!A.split_any) {                      // (cannot happen: a split needs an importance rise)
         atomicAdd(&A.ctr->err_split, 1u);
         return;
     }
@@ -799,6 +842,20 @@ __device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, 
         atomicAdd(&A.fcount[f], (int)nc);
     }
     const uint64_t hid = A.hid[s];
+    if (kGrad) {
+        // the copies carry the particle's score function S (its H, Q and
+        // pending coefficients stay with it) and its visited materials, the
+        // cell it enters included
+        const DProb PR = A.prob[p_id];
+        uint32_t vm = A.vmat[s];
+        const int m1 = A.cell[cell].material;
+        if (m1 >= 0 && A.mat[m1].param_count > 0) vm |= mat_bit(PR, m1);
+        for (uint32_t i = 0; i < nc; ++i) A.rvmat[(t0 + i) % A.ccap] = vm;
+        for (int k = 0; k < PR.param_count; ++k) {
+            const double v = lr_load(A.S + (size_t)k * A.cap + s);
+            for (uint32_t i = 0; i < nc; ++i) A.rS[(size_t)k * A.ccap + (t0 + i) % A.ccap] = v;
+        }
+    }
     uint64_t r = rng;
     for (uint32_t i = 0; i < nc; ++i) {
         r = (r * A.jump40) & kRnMask;

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import slab_analytic as sa
-from mc_testlib import decks, importance_slab, mc, oracle_run
+from mc_testlib import decks, importance_slab, mc, oracle_run, shield_xs_dir
 
 pytestmark = pytest.mark.gpu
 
@@ -75,14 +75,49 @@ def test_split_full_size_vs_analytic():
     assert np.mean(np.abs(z) > 3) < 0.03 and np.abs(z).max() < 4.5
 
 
-def test_split_in_gradient_run_is_rejected():
-    n = 1000
-    deck = decks.slab_deck(2.0, 0.5, 2, n, params=("scatter",))
-    deck = deck.replace('<cell name="bin1" material="slab">', '<cell name="bin1" material="slab" importance="4.0">')
+@pytest.mark.parametrize("drain", [1, -1])
+def test_split_gradient_matches_oracle(drain):
+    """Gradient runs split too: a copy carries the particle's score function S
+    (its H, Q and pending coefficients stay with it) and the family's members
+    add their sums into one, closed out as the history's gradient."""
+    n = 20000
+    deck = decks.slab_deck(2.0, 0.6, 4, n, params=("scatter", "capture"))
+    for k, imp in ((1, 2.0), (2, 3.0), (3, 1.5)):
+        deck = deck.replace(f'<cell name="bin{k}" material="slab">',
+                            f'<cell name="bin{k}" material="slab" importance="{imp!r}">')
     p = mc.Problem.from_string(deck)
-    with pytest.raises(mc.MCError) as e:
-        mc.Engine(p).backward(np.ones(p.n_bins))
-    assert e.value.status == mc.MC_ERR_UNSUPPORTED
+    eng = mc.Engine(p, bank_capacity=1 << 12, drain_divisor=drain)
+    go = np.zeros(p.n_bins)
+    go[0], go[1], go[2:] = 1.0, -0.5, 0.25
+    theta = np.array([1.05, 0.97])
+    g = eng.backward(go, theta=theta)
+    o = oracle_run(p, n, theta=theta, grad_out=go)
+    assert o.flights > 1.5 * n * 2      # the deck does split
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(g.grad, o.grad, rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(g.grad_sigma, o.grad_sigma, rtol=1e-8)
+
+
+def test_split_gradient_shield_matches_oracle():
+    """CE shield with importance 1, 2, 4 in its three layers and 24 parameters
+    (scatter and capture of every nuclide of every layer): copies carry the
+    set of materials the particle has visited as well as S."""
+    n = 3000
+    deck = decks.shield_deck(n, params="24")
+    for (m, _, _), imp in zip(decks.SHIELD_LAYERS, (1.0, 2.0, 4.0)):
+        deck = deck.replace(f'<cell name="{m}" material="{m}">',
+                            f'<cell name="{m}" material="{m}" importance="{imp!r}">')
+    p = mc.Problem.from_string(deck, shield_xs_dir())
+    eng = mc.Engine(p, bank_capacity=1 << 12)
+    rng = np.random.default_rng(5)
+    go = rng.normal(size=p.n_bins)
+    theta = rng.uniform(0.9, 1.1, p.n_params)
+    g = eng.backward(go, theta=theta)
+    o = oracle_run(p, n, theta=theta, grad_out=go)
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(g.grad, o.grad, rtol=1e-9, atol=1e-13)
+    f = eng.forward(theta=theta)
+    assert f.flights == o.flights
 
 
 def test_split_with_collision_and_track_length_estimators():
