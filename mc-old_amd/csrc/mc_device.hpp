@@ -212,11 +212,13 @@ struct KArgs {
     double* famG;
     double* hist;
     // likelihood-ratio state: score function S and H = sum of dS * Q, parameter-
-    // major [param][slot] with row stride cap,
-    // and per slot the scores' gradient weight Q, the pending coefficients cs, cc
+    // major [param][slot] with row stride sstride (cap plus a pad that keeps
+    // the rows of one slot off a power-of-two spacing, so the parameters a
+    // fold touches spread over memory channels), and per slot the scores' gradient weight Q, the pending coefficients cs, cc
     // of dSigma_s/dtheta, dSigma_c/dtheta and the unfolded flight length lpend
     double* S;
     double* H;
+    uint64_t sstride;
     double *Qw, *cs, *cc, *lpend;
     // per slot: the materials with parameters the history has been in (bit
     // m - mat_begin; bit 31: a material of index 31 or above), whose S and H

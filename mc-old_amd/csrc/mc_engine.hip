@@ -536,6 +536,11 @@ void Engine::Impl::upload_static()
 
 constexpr uint64_t kRingPerSlot = 8;
 
+// Row stride (elements) of the parameter-major S and H arrays for a bank of
+// cap slots: cap plus 32 KiB + 256 B, so that the rows of one slot do not sit a
+// power of two apart (a fold updates up to eight of them at once).
+inline uint64_t lr_stride(uint64_t cap) { return cap + 4128; }
+
 void Engine::Impl::ensure_bank(bool grad)
 {
     const uint64_t B = cfg.bank_capacity;
@@ -570,8 +575,8 @@ void Engine::Impl::ensure_bank(bool grad)
         bank_grad = false;
     }
     if (grad && (!bank_grad || realloc_all)) {
-        b_S.alloc(8 * B * (uint64_t)par_stride);
-        b_H.alloc(8 * B * (uint64_t)par_stride);
+        b_S.alloc(8 * lr_stride(B) * (uint64_t)par_stride);
+        b_H.alloc(8 * lr_stride(B) * (uint64_t)par_stride);
         b_Qw.alloc(8 * B);
         b_cs.alloc(8 * B);
         b_cc.alloc(8 * B);
@@ -675,6 +680,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.n_prob = (int32_t)prob.size();
     A.bin_stride = bin_stride;
     A.par_stride = par_stride;
+    A.sstride = lr_stride(bank_cap);
     A.cap = bank_cap;
     A.x = b_x.as<double>(); A.y = b_y.as<double>(); A.z = b_z.as<double>();
     A.u = b_u.as<double>(); A.v = b_v.as<double>(); A.w = b_w.as<double>();
@@ -1003,8 +1009,8 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsq, npp, 0.0);
         // every history's closeout clears the S and H entries it used, so the
         // bank starts each run cleared
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.S, bank_cap * (uint64_t)par_stride, 0.0);
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.H, bank_cap * (uint64_t)par_stride, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.S, lr_stride(bank_cap) * (uint64_t)par_stride, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.H, lr_stride(bank_cap) * (uint64_t)par_stride, 0.0);
         if (split_any)
             hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.famG, bank_cap * (uint64_t)par_stride, 0.0);
     }

@@ -156,7 +156,7 @@ __device__ __forceinline__ void hist_add(const KArgs& A, uint32_t s, int b, doub
 // material with parameters, its extra terms to the coefficients cs, cc:
 //   d ln(score)/d theta = S - lp dSigma_t + [track, coll_end] dSigma_t/Sigma_t + d ln Sigma_x.
 template <bool kGrad>
-__device__ void score_attachment(const KArgs& A, uint32_t s, int att, double l, int cellid, double E, double w,
+__device__ __forceinline__ void score_attachment(const KArgs& A, uint32_t s, int att, double l, int cellid, double E, double w,
                                  const DProb& PR, double lp, double sigt, bool coll_end)
 {
     const int e = A.att_est[att];
@@ -234,7 +234,7 @@ __device__ __forceinline__ void fold_one(const KArgs& A, uint32_t s, int kl, dou
                                          const LrPending& L)
 {
     // parameter-major rows: lanes of a wave touch neighbouring slots
-    const size_t o = (size_t)kl * A.cap + s;
+    const size_t o = (size_t)kl * A.sstride + s;
     if (L.cs != 0.0 || L.cc != 0.0) atomicAdd(&A.H[o], -(L.cs * ds + L.cc * dc));
     double v = -L.lp * (ds + dc);
     v += extra;
@@ -259,16 +259,49 @@ __device__ __forceinline__ uint32_t mat_bit(const DProb& PR, int m)
     return l < 31 ? 1u << l : 1u << 31;
 }
 
-// Surface crossing: fold the flight that ended on the surface, in material M
-// of the cell it leaves (about one flight in a hundred in a shield, so one
-// parameter at a time, which keeps the flight kernel's registers low).
-__device__ void fold_surface(const KArgs& A, uint32_t s, const DProb& PR, const mc_material_desc& M, double E,
+// Surface crossing: fold the flight that ended on the surface, in material m
+// of the cell it leaves (about one flight in a hundred in a shield).  A
+// crossing runs alone in its lane, so its time is its chain of memory round
+// trips: every value a fold needs is loaded before its first update is
+// issued (a material whose nuclides each carry at most one scatter and one
+// capture parameter and whose record holds their sigma_s and sigma_c pairs
+// reads them from that one record).
+__device__ void fold_surface(const KArgs& A, uint32_t s, const DProb& PR, int m, const mc_material_desc& M, double E,
                              XsPoint& P, int gm)
 {
     const LrPending L = lr_pending(A, s);
-    for (int q = 0; q < M.param_count; ++q) {
-        const DSig d = param_dsig(A, M, q, E, P, gm);
-        fold_one(A, s, d.k - PR.param_begin, d.ds, d.dc, 0.0, L);
+    const DMatRec R = A.mrec[m];
+    if (gm >= 0 && R.sc_off >= 0 && A.mfold[m]) {
+        constexpr int kN = 4;                        // nuclides per chunk
+        const double* r = rec_at(A.rec_sc, R.sc_off, R.sc_stride, P);
+        for (int k0 = 0; k0 < M.nuc_count; k0 += kN) {
+            int ks[kN], kc[kN];
+            double ds[kN], dc[kN];
+#pragma unroll
+            for (int t = 0; t < kN; ++t) {
+                if (k0 + t < M.nuc_count) {
+                    const int k = k0 + t, j = M.nuc_begin + k;
+                    const DNucFold F = A.nfold[j];
+                    ks[t] = F.kl_s;
+                    kc[t] = F.kl_c;
+                    const double N = A.mat_dens[j];
+                    ds[t] = xs_pair(r + 2 * k, P) * N;
+                    dc[t] = xs_pair(r + R.c_at + 2 * k, P) * N;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kN; ++t) {
+                if (k0 + t < M.nuc_count) {
+                    if (ks[t] >= 0) fold_one(A, s, ks[t], ds[t], 0.0, 0.0, L);
+                    if (kc[t] >= 0) fold_one(A, s, kc[t], 0.0, dc[t], 0.0, L);
+                }
+            }
+        }
+    } else {
+        for (int q = 0; q < M.param_count; ++q) {
+            const DSig d = param_dsig(A, M, q, E, P, gm);
+            fold_one(A, s, d.k - PR.param_begin, d.ds, d.dc, 0.0, L);
+        }
     }
     if (L.cs != 0.0 || L.cc != 0.0) {
         A.cs[s] = 0.0;
@@ -288,6 +321,51 @@ __device__ __forceinline__ void close_bin(const KArgs& A, double* row, const DPr
     }
 }
 
+// Closes out up to kCloseChunk of a particle's parameters, the t-th being
+// kof(t) (t < n): their S and H entries are all read before the first update
+// is issued.  The gradient term g = Q S - H goes to the workgroup's moments
+// or, for a member of split family fam >= 0, to the family's sum; S and H
+// are cleared for the slot's next particle.
+constexpr int kCloseChunk = 4;
+
+template <class KOf>
+__device__ __forceinline__ void close_chunk(const KArgs& A, const DProb& PR, double* Srow, double* Hrow, size_t ks,
+                                            double Q, bool scored, int fam, int n, KOf kof)
+{
+    int kk[kCloseChunk];
+    double sv[kCloseChunk], hv[kCloseChunk];
+#pragma unroll
+    for (int t = 0; t < kCloseChunk; ++t)
+        if (t < n) kk[t] = kof(t);
+#pragma unroll
+    for (int t = 0; t < kCloseChunk; ++t) {
+        if (t < n && scored) {
+            sv[t] = lr_load(Srow + (size_t)kk[t] * ks);
+            hv[t] = lr_load(Hrow + (size_t)kk[t] * ks);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kCloseChunk; ++t) {
+        if (t < n) {
+            const int k = kk[t];
+            if (scored) {
+                const double g = Q * sv[t] - hv[t];
+                if (g != 0.0) {
+                    if (fam >= 0) {
+                        atomicAdd(&A.famG[(size_t)k * A.cap + fam], g);     // the family's sum
+                    } else {
+                        const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
+                        atomicAdd(&A.pgsum[j], g);
+                        atomicAdd(&A.pgsq[j], g * g);
+                    }
+                }
+                __hip_atomic_store(Hrow + (size_t)k * ks, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            __hip_atomic_store(Srow + (size_t)k * ks, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // Gradient closeout of particle s (Estimator::end_history for the score
 // function): S and H are non-zero only for parameters of the materials the
 // particle has been in.  Their entries are read (when it scored: a history
@@ -304,7 +382,7 @@ __device__ __forceinline__ void grad_closeout(const KArgs& A, uint32_t s, const 
         if (vm != 0) {
             double* Srow = A.S + s;
             double* Hrow = A.H + s;
-            const size_t ks = A.cap;
+            const size_t ks = A.sstride;
             const double Q = A.Qw[s];
             if (kFlush && scored) {
                 // scores still pending (surface scores of the cell the particle ends in)
@@ -320,32 +398,19 @@ __device__ __forceinline__ void grad_closeout(const KArgs& A, uint32_t s, const 
                     }
                 }
             }
-            // the history's gradient sum over its scores gv * S = Q S_end - H
-            auto close_param = [&](int k) {
-                double* ps = Srow + (size_t)k * ks;
-                if (scored) {
-                    double* ph = Hrow + (size_t)k * ks;
-                    const double g = Q * lr_load(ps) - lr_load(ph);
-                    if (g != 0.0) {
-                        if (fam >= 0) {
-                            atomicAdd(&A.famG[(size_t)k * ks + fam], g);     // the family's sum
-                        } else {
-                            const size_t j = (size_t)blockIdx.x * A.par_row + (size_t)(PR.param_begin + k);
-                            atomicAdd(&A.pgsum[j], g);
-                            atomicAdd(&A.pgsq[j], g * g);
-                        }
-                    }
-                    __hip_atomic_store(ph, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                __hip_atomic_store(ps, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            };
+            // the history's gradient sum over its scores gv * S = Q S_end - H,
+            // kCloseChunk parameters at a time with their S and H loaded
+            // before any update is issued (one memory round trip per chunk)
             if (vm & (1u << 31)) {
-                for (int k = 0; k < PR.param_count; ++k) close_param(k);
+                for (int k0 = 0; k0 < PR.param_count; k0 += kCloseChunk)
+                    close_chunk(A, PR, Srow, Hrow, ks, Q, scored, fam, PR.param_count - k0,
+                                [&](int t) { return k0 + t; });
             } else {
                 for (uint32_t b = vm; b != 0; b &= b - 1) {
                     const mc_material_desc M = A.mat[PR.mat_begin + __ffs(b) - 1];
-                    for (int q = 0; q < M.param_count; ++q)
-                        close_param(A.mat_param[M.param_begin + q] - PR.param_begin);
+                    for (int q0 = 0; q0 < M.param_count; q0 += kCloseChunk)
+                        close_chunk(A, PR, Srow, Hrow, ks, Q, scored, fam, M.param_count - q0,
+                                    [&](int t) { return A.mat_param[M.param_begin + q0 + t] - PR.param_begin; });
                 }
             }
         }
@@ -387,7 +452,7 @@ __device__ uint8_t finish_family(const KArgs& A, uint32_t s, const DProb& PR)
 // Returns the status the caller stores in slot s.
 // kFlush: scores may still be pending (not after a collision, which folds them).
 template <bool kGrad, bool kFlush>
-__device__ uint8_t finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
+__device__ __forceinline__ uint8_t finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
                                   double E, uint32_t nfl)
 {
     if (A.split_any && A.fam[s] >= 0) {
@@ -543,7 +608,7 @@ __device__ __forceinline__ void start_copy(const KArgs& A, uint32_t i, uint64_t 
         const int p = A.cprob[k];
         const DProb PR = A.prob[p];
         A.vmat[i] = A.rvmat[k];
-        for (int q = 0; q < PR.param_count; ++q) A.S[(size_t)q * A.cap + i] = A.rS[(size_t)q * A.ccap + k];
+        for (int q = 0; q < PR.param_count; ++q) A.S[(size_t)q * A.sstride + i] = A.rS[(size_t)q * A.ccap + k];
         A.Qw[i] = 0.0;
         A.cs[i] = 0.0;
         A.cc[i] = 0.0;
@@ -852,7 +917,7 @@ __device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, 
         if (m1 >= 0 && A.mat[m1].param_count > 0) vm |= mat_bit(PR, m1);
         for (uint32_t i = 0; i < nc; ++i) A.rvmat[(t0 + i) % A.ccap] = vm;
         for (int k = 0; k < PR.param_count; ++k) {
-            const double v = lr_load(A.S + (size_t)k * A.cap + s);
+            const double v = lr_load(A.S + (size_t)k * A.sstride + s);
             for (uint32_t i = 0; i < nc; ++i) A.rS[(size_t)k * A.ccap + (t0 + i) % A.ccap] = v;
         }
     }
@@ -890,7 +955,7 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
                 // a shared grid implies tables, so the lookup kernel stored the interval
                 XsPoint P;
                 const int gm = material_point_cached(A, m, s, E, P);
-                fold_surface(A, s, PR, M, E, P, gm);
+                fold_surface(A, s, PR, m, M, E, P, gm);
             }
         }
     }
