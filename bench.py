@@ -10,6 +10,9 @@ Also reported on the same JSON line:
   batched_vs_plain  the 4096-slab sweep in one batched launch sequence vs one
                     problem per launch sequence (plain timed on a seeded subset)
   bwd_fwd           likelihood-ratio backward cost / forward cost, 24-parameter shield
+  box_variants      64 design variants of the 3D box shield in one batched engine:
+                    detector response and its 80-parameter gradient per variant
+                    (--box-histories total, 1e8 by default; the full workload is 1e9)
   xs_lookup         the standalone lookup kernel against HBM peak
   roofline          the dominant event-loop kernel (algorithmic bytes per particle
                     event, device time from per-kernel HIP events)
@@ -197,6 +200,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=float, default=1e8)
     ap.add_argument("--cpu-sample", type=int, default=100000)
+    ap.add_argument("--box-histories", type=float, default=1e8,
+                    help="total histories of the 64-variant 3D box run (the full workload is 1e9)")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
@@ -413,6 +418,32 @@ def main():
                           "params": int(e3.n_params), "grad_out": "ones over all 201 tally bins",
                           "dT_dtheta_max_rel_sigma": float(np.max(b3.grad_sigma / np.maximum(np.abs(b3.grad), 1e-300)))}
         log(f"bwd/fwd: {tbw:.3f}/{tf:.3f} = {tbw / tf:.2f}")
+
+        # ------------------------------------------------ 3D box: 64 design variants batched
+        nv = 64
+        nbox = int(args.box_histories) // nv
+        thetas = decks.box_variant_thetas(nv)
+        pbs = [mc.Problem.from_string(decks.box_deck(nbox), xs) for _ in range(nv)]
+        e4 = mc.Engine(pbs, use_graph=G)
+        th4 = thetas.ravel()
+        go4 = np.ones(e4.n_bins)
+        e4.forward(n_histories=max(1, nbox // 20), theta=th4)
+        e4.backward(go4, n_histories=max(1, nbox // 20), theta=th4)
+        t = time.perf_counter()
+        f4 = e4.forward(theta=th4)
+        tf4 = time.perf_counter() - t
+        t = time.perf_counter()
+        b4 = e4.backward(go4, theta=th4)
+        tb4 = time.perf_counter() - t
+        out["box_variants"] = {"variants": nv, "histories": nv * nbox, "params_per_variant": int(e4.n_params) // nv,
+                               "forward_s": tf4, "forward_hist_per_s": nv * nbox / tf4,
+                               "backward_s": tb4, "backward_hist_per_s": nv * nbox / tb4, "ratio": tb4 / tf4,
+                               "flights": int(f4.flights),
+                               "response_rel_sigma_median": float(np.median(f4.uncer / np.maximum(f4.mean, 1e-300))),
+                               "workload": "20-cell box shield, spherical detector 1 m beyond, per-(cell, nuclide) "
+                                           "scale factors in [0.8, 1.2] per variant; one batched engine"}
+        log(f"box: {nv} variants x {nbox} histories, forward {tf4:.3f} s, backward {tb4:.3f} s "
+            f"({tb4 / tf4:.2f}x)")
 
     print(json.dumps(out), flush=True)
 
