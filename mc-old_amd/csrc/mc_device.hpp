@@ -177,6 +177,9 @@ struct KArgs {
     int32_t split_any;            // some problem can split (importance rises between cells)
     int32_t pad_split;
     const uint64_t* run_hb;       // per problem first key of this run (n_prob + 1)
+    // per run: {shift, n, then for j = 0..n the problem holding key
+    // min(j << shift, last key)}; n <= 65536 blocks of 2^shift keys
+    const int32_t* hb_idx;
     int32_t n_prob;
     int32_t bin_stride;           // hist row length (max bins per problem)
     int32_t par_stride;           // rows of S and H (max params per problem)

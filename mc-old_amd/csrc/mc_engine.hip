@@ -47,6 +47,9 @@ __global__ void k_selftest_math(const double* in, double* out, uint64_t n);
     } while (0)
 
 // ----------------------------------------------------------- device buffers
+// Blocks of the per-run key-to-problem index.
+constexpr uint32_t kHbIdxMax = 65536;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -124,7 +127,8 @@ struct Engine::Impl {
         d_mat_ns, d_mat_nc, d_mat_tconst, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nfold, d_mfold, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
-        d_run_hb;
+        d_run_hb, d_hb_idx;
+    std::vector<int32_t> h_hb_idx;    // host copy of the run's key-to-problem index
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
         b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, b_vmat, q_S, q_A, q_C, d_ctr, d_wgcnt, b_mat;
@@ -524,6 +528,7 @@ void Engine::Impl::upload_static()
     upload(d_jump, jump, stream);
     if (split_any) d_wgheld.alloc(sizeof(uint32_t) * 8192);
     d_run_hb.alloc(sizeof(uint64_t) * (prob.size() + 1));
+    d_hb_idx.alloc(sizeof(int32_t) * (kHbIdxMax + 3));
     t_sum.alloc(sizeof(double) * std::max(1, n_bins));
     t_sq.alloc(sizeof(double) * std::max(1, n_bins));
     t_gout.alloc(sizeof(double) * std::max(1, n_bins));
@@ -677,6 +682,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
         A.ccap = ring_cap;
     }
     A.run_hb = d_run_hb.as<uint64_t>();
+    A.hb_idx = d_hb_idx.as<int32_t>();
     A.n_prob = (int32_t)prob.size();
     A.bin_stride = bin_stride;
     A.par_stride = par_stride;
@@ -763,7 +769,9 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     }
     mark(3, 1);
     mark(4, 0);
-    if (has_tables) {
+    if (!has_tables && !grad) {
+        // one-speed forward runs: the flight kernel collided
+    } else if (has_tables) {
         if (grad) hipLaunchKernelGGL((k_collision<true, true>), g, b, 0, st, A);
         else hipLaunchKernelGGL((k_collision<true, false>), g, b, 0, st, A);
     } else {
@@ -983,6 +991,26 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     upload(d_nfold, nfold, stream);
     upload(d_mfold, mfold, stream);
     HIP_CHECK(hipMemcpyAsync(d_run_hb.p, hb.data(), sizeof(uint64_t) * (np + 1), hipMemcpyHostToDevice, stream));
+    {
+        // key -> problem index: blocks of 2^sh keys, at most kHbIdxMax of them;
+        // a key's problem lies between those of its block's two ends
+        const uint64_t total = hb[np];
+        uint32_t sh = 0;
+        while ((total >> sh) >= kHbIdxMax) ++sh;
+        const uint32_t nj = (uint32_t)(total >> sh) + 1;
+        std::vector<int32_t>& idx = h_hb_idx;
+        idx.assign(nj + 3, 0);
+        idx[0] = (int32_t)sh;
+        idx[1] = (int32_t)nj;
+        int p = 0;
+        for (uint32_t j = 0; j <= nj; ++j) {
+            uint64_t key = (uint64_t)j << sh;
+            if (total > 0 && key > total - 1) key = total - 1;
+            while (p + 1 < (int)np && hb[p + 1] <= key) ++p;
+            idx[2 + j] = p;
+        }
+        HIP_CHECK(hipMemcpyAsync(d_hb_idx.p, idx.data(), sizeof(int32_t) * (nj + 3), hipMemcpyHostToDevice, stream));
+    }
     if (grad) {
         std::vector<double> g(gout, gout + n_bins);
         HIP_CHECK(hipMemcpyAsync(t_gout.p, g.data(), sizeof(double) * n_bins, hipMemcpyHostToDevice, stream));

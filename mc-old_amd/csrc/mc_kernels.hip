@@ -17,7 +17,9 @@
 //                     conditions, cell search, surface tallies, importance
 //                     split or roulette, weight roulette (general.cpp:89-115,
 //                     population_control.cpp:9-49); collisions go to the
-//                     workgroup's own segment of queue C
+//                     workgroup's own segment of queue C (one-speed forward
+//                     runs collide right after the flight, in the same thread,
+//                     and launch no k_collision)
 //   k_collision       per workgroup, the segment its flight workgroup filled:
 //                     collision tallies, implicit capture, nuclide selection,
 //                     elastic scatter, weight roulette (general.cpp:121-163)
@@ -89,7 +91,11 @@ __device__ __forceinline__ void wg_prefix(const uint32_t* cnt, uint32_t& a_befor
 
 __device__ __forceinline__ int find_problem(const KArgs& A, uint64_t hid)
 {
-    int lo = 0, hi = A.n_prob - 1;   // largest p with run_hb[p] <= hid
+    // the run's index gives the problems of the 2^shift-key block holding hid
+    const int32_t* t = A.hb_idx;
+    uint64_t j = hid >> t[0];
+    if (j >= (uint64_t)t[1]) j = (uint64_t)t[1] - 1;
+    int lo = t[2 + j], hi = t[3 + j];   // largest p with run_hb[p] <= hid
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (A.run_hb[mid] <= hid) lo = mid;
@@ -691,10 +697,15 @@ __device__ __forceinline__ bool refill_free(const KArgs& A, uint8_t st, uint32_t
     return st == ST_DEAD || st == ST_EMPTY || (st == ST_HELD && (cmode || A.fcount[i] == 0));
 }
 
+// Slots a refill workgroup lists before it starts their particles (at least
+// twice the largest workgroup).
+constexpr uint32_t kRefillList = 2048;
+
 template <bool kGrad>
 __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
 {
     __shared__ uint32_t lds[64];
+    __shared__ uint32_t s_list[kRefillList];
     __shared__ uint64_t s_jb;
     __shared__ int s_pb;
     const uint32_t cap = A.ctr->active_cap;
@@ -752,6 +763,11 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
         const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
         const uint64_t below = (1ull << lane) - 1ull;
         uint64_t fbase = free_before, lbase = live_before;
+        // the slots that get a particle are listed in LDS in rank order and
+        // started by the whole block at once (one dependent chain of loads per
+        // thread per flush, not one per tile); entry k has rank list_f0 + k
+        uint32_t listed = 0;
+        uint64_t list_f0 = free_before;
         // statuses of the next tile are loaded ahead of this tile's scan
         uint8_t st_next = lo + threadIdx.x < hi ? A.status[lo + threadIdx.x] : ST_SURF;
         find_seed();
@@ -787,14 +803,27 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
                 if (!R.ident) A.qA[act] = i;
             } else if (fr) {
                 if (fidx < R.lim) {
-                    refill_slot<kGrad>(A, R, i, fidx, p);
+                    s_list[fidx - list_f0] = i;
                     if (!R.ident) A.qA[act] = i;
                 } else if (st == ST_DEAD) {
                     A.status[i] = ST_EMPTY;
                 }
             }
+            // slots of this tile that get a particle (the same in every thread)
+            const uint64_t got = R.lim > fbase ? (R.lim - fbase < tf ? R.lim - fbase : (uint64_t)tf) : 0;
+            listed += (uint32_t)got;
             fbase += tf;
             lbase += tl;
+            if (listed + blockDim.x > kRefillList || base + blockDim.x >= hi) {
+                __syncthreads();
+                for (uint32_t k = threadIdx.x; k < listed; k += blockDim.x) {
+                    int pk = R.pb;
+                    refill_slot<kGrad>(A, R, s_list[k], list_f0 + k, pk);
+                }
+                __syncthreads();
+                list_f0 += listed;
+                listed = 0;
+            }
         }
     } else {
         const uint32_t per = (hi - lo + blockDim.x - 1) / blockDim.x;
@@ -1105,9 +1134,15 @@ __device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
 // matter, every crossing is independent) and crossed by the whole block
 // once the queue fills or the range ends, so the rare crossing does not hold
 // a wave's other lanes and its registers stay out of the flight loop.
+template <bool kTables, bool kGrad, bool kInlineFreeGas = true>
+__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s);
+
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
+    // one-speed forward runs collide in the same pass (a collision there is a
+    // few dozen instructions; a kernel of its own would re-read the state)
+    constexpr bool kFuse = !kTables && !kGrad;
     constexpr uint32_t kXq = 2048;      // >= the largest workgroup
     __shared__ uint32_t xq[kXq];
     __shared__ uint32_t xn, cn, sn;
@@ -1126,11 +1161,12 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         if (i < hi) {
             bool crossed;
             st = flight_one<kTables, kGrad, true>(A, s, crossed);
+            if (kFuse && st == ST_COLL) collision_one<kTables, false>(A, s);
         }
         // collisions go to this block's segment of queue C (slots of a wave in
         // lane order, one LDS atomic per wave); crossings to the LDS queue
         const uint32_t ci = wave_append(st == ST_COLL, &cn);
-        if (st == ST_COLL) A.qC[lo + ci] = s;
+        if (!kFuse && st == ST_COLL) A.qC[lo + ci] = s;
         const uint32_t xi = wave_append(st == ST_SURF, &xn);
         if constexpr (kGrad) {
             // gradient runs: crossings go to this block's segment of queue S
@@ -1214,7 +1250,7 @@ __device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
     return f;
 }
 
-template <bool kTables, bool kGrad, bool kInlineFreeGas = true>
+template <bool kTables, bool kGrad, bool kInlineFreeGas>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
 {
     const int c = A.pcell[s];
