@@ -344,11 +344,13 @@ def main():
         probs = [mc.Problem.from_string(decks.slab_deck(float(l), float(cc), 0, int(k)))
                  for l, cc, k in zip(L, c, N)]
         beng = mc.Engine(probs, use_graph=G)
-        # the autotuner picks workgroup shape, bank capacity and drain threshold
-        # on a 10 % sample (every problem keeps its share of histories)
+        # the autotuner picks workgroup shape, bank capacity, drain threshold and
+        # flights per particle per iteration on a 10 % sample (every problem
+        # keeps its share of histories)
         tune = beng.autotune(int(N.sum() // 10))
         tuned = {"block": tune.block, "grid_per_cu": tune.grid_per_cu, "bank_capacity": int(tune.bank_capacity),
-                 "drain_divisor": tune.drain_divisor, "sample_hist_per_s": tune.histories_per_second}
+                 "drain_divisor": tune.drain_divisor, "events_per_launch": tune.events_per_launch,
+                 "sample_hist_per_s": tune.histories_per_second}
         log(f"autotune: {tuned}")
         beng.forward(n_histories=np.maximum(2, N // 1000))   # warm-up (graph capture)
         t = time.perf_counter()
@@ -360,7 +362,8 @@ def main():
         # plain: the same kernels and launch shape, one problem at a time with
         # ordinary launches (no graph replay, no batching)
         peng2 = mc.Engine([probs[i] for i in sub], use_graph=False, block=tune.block, grid_per_cu=tune.grid_per_cu,
-                          bank_capacity=int(tune.bank_capacity), drain_divisor=tune.drain_divisor)
+                          bank_capacity=int(tune.bank_capacity), drain_divisor=tune.drain_divisor,
+                          events_per_launch=tune.events_per_launch)
         peng2.forward(n_histories=np.maximum(2, N[sub] // 1000), mode=mc.MC_RUN_PLAIN)
         t = time.perf_counter()
         peng2.forward(mode=mc.MC_RUN_PLAIN)
@@ -369,7 +372,7 @@ def main():
 
         def plain_rate(**kw):
             e = mc.Engine([probs[i] for i in sub], block=tune.block, grid_per_cu=tune.grid_per_cu,
-                          bank_capacity=int(tune.bank_capacity), **kw)
+                          bank_capacity=int(tune.bank_capacity), events_per_launch=tune.events_per_launch, **kw)
             e.forward(n_histories=np.maximum(2, N[sub] // 1000), mode=mc.MC_RUN_PLAIN)
             t0 = time.perf_counter()
             e.forward(mode=mc.MC_RUN_PLAIN)

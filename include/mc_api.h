@@ -189,7 +189,10 @@ typedef struct mc_engine_cfg {
     int32_t drain_divisor;   /* once every history has started and at most
                                 bank/drain_divisor particles live, finish them
                                 in one drain launch (0: 1, < 0: never)        */
-    int32_t pad;
+    int32_t events_per_launch; /* one-speed forward runs: flights, each with its
+                                collision, a particle takes per event iteration
+                                before the bank is refilled (0: 4; 1: one);
+                                runs that can split take one                  */
 } mc_engine_cfg;
 
 /* Per-kernel device time of the event loop, recorded when cfg.profile = 1:
@@ -273,9 +276,11 @@ typedef struct mc_tune_result {
     int32_t block, grid_per_cu, iters_per_graph, drain_divisor;
     uint64_t bank_capacity;
     double histories_per_second;
+    int32_t events_per_launch, pad;
 } mc_tune_result;
 /* Times short batched runs over a grid of launch shapes (workgroup size,
- * workgroups per CU, bank capacity, drain threshold) on a sample of the
+ * workgroups per CU, bank capacity, drain threshold, and for engines without
+ * cross-section tables the flights per particle per iteration) on a sample of the
  * engine's histories, every problem keeping its share, and installs the
  * fastest.  No reference counterpart (the reference has no launch shapes). */
 mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* out);

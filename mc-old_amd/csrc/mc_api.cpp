@@ -254,17 +254,23 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
         const int grids[] = {2, 4};
         const uint64_t banks[] = {1ull << 19, 1ull << 20, 1ull << 21};
         const int drains[] = {1, -1};
+        // flights per particle per iteration matter only where the flight
+        // kernel collides (engines without cross-section tables)
+        std::vector<int> events = {base.events_per_launch};
+        if (!E.has_tables()) events = {1, 2, 4, 8};
         for (uint64_t B : banks) {
             if (B > 4 * sampled && B != banks[0]) continue;
             for (int blk : blocks)
                 for (int gpc : grids)
-                    for (int dd : drains) {
+                    for (int dd : drains)
+                    for (int ev : events) {
                         mc_engine_cfg c = base;
                         c.block = blk;
                         c.grid_per_cu = gpc;
                         c.bank_capacity = B;
                         c.iters_per_graph = 16;
                         c.drain_divisor = dd;
+                        c.events_per_launch = ev;
                         c.use_graph = base.use_graph;   // tune the launch mode the engine uses
                         c.profile = 0;
                         E.set_cfg(c);
@@ -282,6 +288,7 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
         out->drain_divisor = best.drain_divisor;
         out->bank_capacity = best.bank_capacity;
         out->histories_per_second = best_rate;
+        out->events_per_launch = best.events_per_launch;
     });
 }
 

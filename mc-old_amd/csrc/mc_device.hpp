@@ -67,6 +67,7 @@ struct Counters {
                                   // the identity (not written; kernels use the slot index)
     uint64_t chead, ctail;        // copy ring: split copies waiting for a slot are [chead, ctail)
     uint32_t cplaced, err_ring;   // copies the last refill placed; ring overflows
+    uint64_t xfl;                 // flights taken beyond the first of an iteration (included in tA)
 };
 
 // Entry i of queue A.
@@ -180,6 +181,11 @@ struct KArgs {
     // per run: {shift, n, then for j = 0..n the problem holding key
     // min(j << shift, last key)}; n <= 65536 blocks of 2^shift keys
     const int32_t* hb_idx;
+    // one-speed forward runs: flights (each with its collision) a particle
+    // takes per event iteration; per flight workgroup, the flights beyond the
+    // first of each particle, folded into the counters by the next refill
+    uint32_t events_per_launch;
+    uint64_t* wg_xfl;
     int32_t n_prob;
     int32_t bin_stride;           // hist row length (max bins per problem)
     int32_t par_stride;           // rows of S and H (max params per problem)

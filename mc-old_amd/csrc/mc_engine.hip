@@ -131,7 +131,8 @@ struct Engine::Impl {
     std::vector<int32_t> h_hb_idx;    // host copy of the run's key-to-problem index
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
-        b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, b_vmat, q_S, q_A, q_C, d_ctr, d_wgcnt, b_mat;
+        b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, b_vmat, q_S, q_A, q_C, d_ctr, d_wgcnt, b_mat,
+        d_wgxfl;
     // split families and the copy ring (problems that can split only)
     DevBuf b_fam, b_fcount, d_wgheld, r_S, r_vmat, b_famG, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
         r_nfl;
@@ -536,6 +537,7 @@ void Engine::Impl::upload_static()
     t_gsq.alloc(sizeof(double) * std::max(1, n_params));
     d_ctr.alloc(sizeof(Counters));
     d_wgcnt.alloc(sizeof(uint32_t) * 2 * 8192);   // per-workgroup counts (grid <= 8192 blocks)
+    d_wgxfl.alloc(sizeof(uint64_t) * 8192);
     HIP_CHECK(hipStreamSynchronize(stream));
 }
 
@@ -705,6 +707,10 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.tcount = b_tcount.as<uint8_t>();
     A.tbins = b_tbins.as<uint32_t>();
     A.wg_cnt = d_wgcnt.as<uint32_t>();
+    A.wg_xfl = d_wgxfl.as<uint64_t>();
+    // a run that can split takes one flight per particle per iteration: its
+    // copies wait in the ring for free slots, which appear once per iteration
+    A.events_per_launch = split_any ? 1u : (uint32_t)cfg.events_per_launch;
     A.hist = b_hist.as<double>();
     A.S = grad ? b_S.as<double>() : nullptr;
     A.H = grad ? b_H.as<double>() : nullptr;
@@ -831,7 +837,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                            (uint64_t)h_ctr->nA * (uint64_t)cfg.drain_divisor <= (uint64_t)h_active_cap;
         if (drain) {
             // the roll-over in the refill below adds the last iteration's counts
-            const uint64_t tA0 = h_ctr->tA + h_ctr->nA, tC0 = h_ctr->tC + h_ctr->nC;
+            const uint64_t tA0 = h_ctr->tA + h_ctr->nA, tC0 = h_ctr->tC + h_ctr->nC, x0 = h_ctr->xfl;
             refill();
             if (profile) HIP_CHECK(hipEventRecord(kev[2 * kDrainSlot], stream));
             const uint32_t kMaxEvents = 1u << 14;
@@ -848,7 +854,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, stream, A.ctr);
             refill();
             read_ctr();
-            drain_flights += h_ctr->tA - tA0;
+            drain_flights += h_ctr->tA - tA0 - (h_ctr->xfl - x0);
             drain_coll += h_ctr->tC - tC0;
             if (profile) {
                 float ms = 0.f;
@@ -1055,6 +1061,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         h_active_cap = c.active_cap;
         HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.status, bank_cap, (uint8_t)ST_EMPTY);
+        hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, (uint8_t*)A.wg_xfl, sizeof(uint64_t) * 8192, (uint8_t)0);
         if (split_any) {
             hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, stream, A.fam, bank_cap, -1);
             hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, stream, A.fcount, bank_cap, 0);
@@ -1135,6 +1142,9 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     if (cfg.iters_per_graph == 0) cfg.iters_per_graph = 16;
     if (cfg.bank_capacity == 0) cfg.bank_capacity = 1ull << 21;
     if (cfg.drain_divisor == 0) cfg.drain_divisor = 1;
+    if (cfg.events_per_launch == 0) cfg.events_per_launch = 4;
+    if (cfg.events_per_launch < 1 || cfg.events_per_launch > 64)
+        throw Error(MC_ERR_ARGUMENT, "events_per_launch must be in 1..64");
     if (cfg.block % 64 || cfg.block > 1024) throw Error(MC_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
     if (cfg.bank_capacity > (1ull << 31)) throw Error(MC_ERR_CAPACITY, "bank capacity above 2^31");
     d->cfg = cfg;
@@ -1263,7 +1273,8 @@ void Engine::kernel_stats(mc_kernel_stats* out, bool reset)
 void Engine::set_cfg(const mc_engine_cfg& c)
 {
     if ((int64_t)d->n_cu * c.grid_per_cu > 8192 || c.grid_per_cu < 1 || c.block % 64 || c.block < 64 ||
-        c.block > 1024 || c.bank_capacity == 0 || c.bank_capacity > (1ull << 31))
+        c.block > 1024 || c.bank_capacity == 0 || c.bank_capacity > (1ull << 31) || c.events_per_launch < 0 ||
+        c.events_per_launch > 64)
         throw Error(MC_ERR_ARGUMENT, "invalid launch configuration");
     d->cfg.profile = c.profile;
     d->cfg.block = c.block;
@@ -1271,6 +1282,7 @@ void Engine::set_cfg(const mc_engine_cfg& c)
     d->cfg.iters_per_graph = c.iters_per_graph;
     d->cfg.use_graph = c.use_graph;
     d->cfg.drain_divisor = c.drain_divisor == 0 ? 1 : c.drain_divisor;
+    d->cfg.events_per_launch = c.events_per_launch == 0 ? 4 : c.events_per_launch;
     if (c.bank_capacity != d->cfg.bank_capacity) d->cfg.bank_capacity = c.bank_capacity;
     ++d->generation;
 }
@@ -1278,6 +1290,7 @@ void Engine::set_cfg(const mc_engine_cfg& c)
 mc_engine_cfg Engine::cfg() const { return d->cfg; }
 
 int Engine::n_problems() const { return (int)d->prob.size(); }
+bool Engine::has_tables() const { return d->has_tables; }
 uint64_t Engine::deck_histories(int p) const { return d->probs[p]->n_histories; }
 
 void selftest_math(const double* in, double* out, uint64_t n)

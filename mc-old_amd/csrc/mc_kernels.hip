@@ -526,6 +526,25 @@ __global__ void __launch_bounds__(1024) k_refill_count(KArgs A)
         c->nS = 0;
         c->nC = 0;
     }
+    if (A.events_per_launch > 1 && blockIdx.x == 0) {
+        // flights the last flight launch took beyond the first per particle
+        // (the flight kernel runs on this grid, one entry per workgroup)
+        __shared__ unsigned long long xs;
+        if (threadIdx.x == 0) xs = 0;
+        unsigned long long x = 0;
+        for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            x += A.wg_xfl[b];
+            A.wg_xfl[b] = 0;
+        }
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        __syncthreads();
+        if (lane_id() == 0 && x != 0) atomicAdd(&xs, x);
+        __syncthreads();
+        if (threadIdx.x == 0 && xs != 0) {
+            A.ctr->tA += xs;
+            A.ctr->xfl += xs;
+        }
+    }
     uint32_t lo, hi;
     block_range(A.ctr->active_cap, lo, hi);
     uint32_t live = 0, freec = 0, held = 0;
@@ -1140,16 +1159,22 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s);
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
-    // one-speed forward runs collide in the same pass (a collision there is a
-    // few dozen instructions; a kernel of its own would re-read the state)
+    // One-speed forward runs collide in the same pass (a collision there is a
+    // few dozen instructions; a kernel of its own would re-read the state),
+    // and a particle that survives its collision flies again, up to
+    // events_per_launch flights, before the bank is refilled.  Every particle
+    // still takes its own events in order; only how many of them one
+    // iteration holds changes.
     constexpr bool kFuse = !kTables && !kGrad;
+    uint32_t extra = 0;                 // flights beyond the first, this thread
     constexpr uint32_t kXq = 2048;      // >= the largest workgroup
     __shared__ uint32_t xq[kXq];
     __shared__ uint32_t xn, cn, sn;
+    __shared__ unsigned long long xf;
     const uint32_t n = A.ctr->nA;
     uint32_t lo, hi;
     block_range(n, lo, hi);
-    if (threadIdx.x == 0) { xn = 0; cn = 0; sn = 0; }
+    if (threadIdx.x == 0) { xn = 0; cn = 0; sn = 0; xf = 0; }
     __syncthreads();
     const uint32_t ident = A.ctr->qa_ident;
     uint32_t s_next = lo + threadIdx.x < hi ? qa_at(A.qA, ident, lo + threadIdx.x) : 0u;   // read one item ahead
@@ -1161,7 +1186,12 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         if (i < hi) {
             bool crossed;
             st = flight_one<kTables, kGrad, true>(A, s, crossed);
-            if (kFuse && st == ST_COLL) collision_one<kTables, false>(A, s);
+            for (uint32_t e = 1; kFuse && st == ST_COLL; ++e) {
+                st = collision_one<kTables, false>(A, s);
+                if (st != ST_ALIVE || e >= A.events_per_launch) break;
+                st = flight_one<kTables, kGrad, true>(A, s, crossed);
+                ++extra;
+            }
         }
         // collisions go to this block's segment of queue C (slots of a wave in
         // lane order, one LDS atomic per wave); crossings to the LDS queue
@@ -1190,6 +1220,13 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         __syncthreads();
         if (threadIdx.x == 0) sn = xn;
         __syncthreads();
+    }
+    if constexpr (kFuse) {
+        unsigned long long w = extra;
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o);
+        if (lane_id() == 0 && w != 0) atomicAdd(&xf, w);
+        __syncthreads();
+        if (threadIdx.x == 0 && xf != 0) A.wg_xfl[blockIdx.x] += xf;   // this block's own entry
     }
     // the collision kernel's workgroup b takes this block's segment
     if (threadIdx.x == 0) {
