@@ -345,9 +345,10 @@ def main():
                  for l, cc, k in zip(L, c, N)]
         beng = mc.Engine(probs, use_graph=G)
         # the autotuner picks workgroup shape, bank capacity, drain threshold and
-        # flights per particle per iteration on a 10 % sample (every problem
-        # keeps its share of histories)
-        tune = beng.autotune(int(N.sum() // 10))
+        # flights per particle per iteration on a 25 % sample (every problem
+        # keeps its share of histories; a smaller sample favours smaller banks
+        # than the full sweep wants)
+        tune = beng.autotune(int(N.sum() // 4))
         tuned = {"block": tune.block, "grid_per_cu": tune.grid_per_cu, "bank_capacity": int(tune.bank_capacity),
                  "drain_divisor": tune.drain_divisor, "events_per_launch": tune.events_per_launch,
                  "sample_hist_per_s": tune.histories_per_second}

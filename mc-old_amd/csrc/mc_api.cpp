@@ -257,7 +257,7 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
         // flights per particle per iteration matter only where the flight
         // kernel collides (engines without cross-section tables)
         std::vector<int> events = {base.events_per_launch};
-        if (!E.has_tables()) events = {1, 2, 4, 8};
+        if (!E.has_tables()) events = {1, 2, 4, 6, 8};
         for (uint64_t B : banks) {
             if (B > 4 * sampled && B != banks[0]) continue;
             for (int blk : blocks)

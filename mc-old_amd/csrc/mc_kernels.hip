@@ -1185,13 +1185,15 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         uint8_t st = ST_DEAD;
         if (i < hi) {
             bool crossed;
-            st = flight_one<kTables, kGrad, true>(A, s, crossed);
-            for (uint32_t e = 1; kFuse && st == ST_COLL; ++e) {
-                st = collision_one<kTables, false>(A, s);
-                if (st != ST_ALIVE || e >= A.events_per_launch) break;
+            uint32_t nfl = 0;
+            for (;;) {
                 st = flight_one<kTables, kGrad, true>(A, s, crossed);
-                ++extra;
+                ++nfl;
+                if (!kFuse || st != ST_COLL) break;
+                st = collision_one<kTables, false, false>(A, s);
+                if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
             }
+            extra += nfl - 1;
         }
         // collisions go to this block's segment of queue C (slots of a wave in
         // lane order, one LDS atomic per wave); crossings to the LDS queue
