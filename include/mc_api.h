@@ -189,15 +189,17 @@ typedef struct mc_engine_cfg {
     int32_t drain_divisor;   /* once every history has started and at most
                                 bank/drain_divisor particles live, finish them
                                 in one drain launch (0: 1, < 0: never)        */
-    int32_t events_per_launch; /* one-speed forward runs: flights, each with its
+    int32_t events_per_launch; /* forward runs: flights, each with its
                                 collision, a particle takes per event iteration
-                                before the bank is refilled (0: 4; 1: one);
+                                before the bank is refilled (0: 8; 1: one);
                                 runs that can split take one                  */
 } mc_engine_cfg;
 
 /* Per-kernel device time of the event loop, recorded when cfg.profile = 1:
  * kernels in order refill_count, refill_scatter, xs_lookup, flight (with the
- * surface crossings), collision, drain.                                    */
+ * surface crossings; in forward runs also the collisions and the further
+ * flights of each particle), collision (gradient runs), drain.  Items of the
+ * flight kernel are particles per launch, not flights.                      */
 #define MC_N_KERNELS 6
 typedef struct mc_kernel_stats {
     uint64_t launches[MC_N_KERNELS];
@@ -279,8 +281,8 @@ typedef struct mc_tune_result {
     int32_t events_per_launch, pad;
 } mc_tune_result;
 /* Times short batched runs over a grid of launch shapes (workgroup size,
- * workgroups per CU, bank capacity, drain threshold, and for engines without
- * cross-section tables the flights per particle per iteration) on a sample of the
+ * workgroups per CU, bank capacity, drain threshold, flights per particle per
+ * iteration) on a sample of the
  * engine's histories, every problem keeping its share, and installs the
  * fastest.  No reference counterpart (the reference has no launch shapes). */
 mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* out);

@@ -254,10 +254,8 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
         const int grids[] = {2, 4};
         const uint64_t banks[] = {1ull << 19, 1ull << 20, 1ull << 21};
         const int drains[] = {1, -1};
-        // flights per particle per iteration matter only where the flight
-        // kernel collides (engines without cross-section tables)
-        std::vector<int> events = {base.events_per_launch};
-        if (!E.has_tables()) events = {1, 2, 4, 6, 8};
+        // flights per particle per iteration (forward runs)
+        const std::vector<int> events = {1, 2, 4, 6, 8};
         for (uint64_t B : banks) {
             if (B > 4 * sampled && B != banks[0]) continue;
             for (int blk : blocks)

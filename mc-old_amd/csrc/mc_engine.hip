@@ -775,8 +775,8 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     }
     mark(3, 1);
     mark(4, 0);
-    if (!has_tables && !grad) {
-        // one-speed forward runs: the flight kernel collided
+    if (!grad) {
+        // forward runs: the flight kernel collided
     } else if (has_tables) {
         if (grad) hipLaunchKernelGGL((k_collision<true, true>), g, b, 0, st, A);
         else hipLaunchKernelGGL((k_collision<true, false>), g, b, 0, st, A);
@@ -876,10 +876,12 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                 launch_iteration(A, grad, stream, kev.data());
                 HIP_CHECK(hipEventSynchronize(kev[2 * kIterKernels - 1]));
                 for (int j = 0; j < kIterKernels; ++j) {
+                    // no lookup kernel without tables, no collision kernel in forward runs
+                    if ((j == 2 && !has_tables) || (j == 4 && !grad)) continue;
                     float ms = 0.f;
                     HIP_CHECK(hipEventElapsedTime(&ms, kev[2 * j], kev[2 * j + 1]));
                     kstats.seconds[j] += 1e-3 * ms;
-                    kstats.launches[j] += (j == 2 && !has_tables) ? 0 : 1;
+                    kstats.launches[j] += 1;
                 }
                 kstats.items[0] += h_active_cap;
                 kstats.items[1] += h_active_cap;
@@ -895,11 +897,13 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                                          " s (MC_MAX_SECONDS); live particles: " + std::to_string(h_ctr->nA));
     }
     if (profile) {
-        // event counts: flight (and lookup) saw every live particle of the
-        // iterations, then collisions; the drain's flights are its own
-        const uint64_t it_flights = h_ctr->tA - drain_flights;
-        kstats.items[2] += has_tables ? it_flights : 0;
-        kstats.items[3] += it_flights;
+        // event counts: the lookup and flight kernels saw every live particle
+        // of the iterations (forward flights beyond a particle's first in a
+        // launch are not items of their own), then collisions; the drain's
+        // flights are its own
+        const uint64_t it_particles = h_ctr->tA - drain_flights - h_ctr->xfl;
+        kstats.items[2] += has_tables ? it_particles : 0;
+        kstats.items[3] += it_particles;
         kstats.items[4] += h_ctr->tC - drain_coll;
         kstats.items[kDrainSlot] += drain_flights;
     }
@@ -1142,7 +1146,7 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     if (cfg.iters_per_graph == 0) cfg.iters_per_graph = 16;
     if (cfg.bank_capacity == 0) cfg.bank_capacity = 1ull << 21;
     if (cfg.drain_divisor == 0) cfg.drain_divisor = 1;
-    if (cfg.events_per_launch == 0) cfg.events_per_launch = 4;
+    if (cfg.events_per_launch == 0) cfg.events_per_launch = 8;
     if (cfg.events_per_launch < 1 || cfg.events_per_launch > 64)
         throw Error(MC_ERR_ARGUMENT, "events_per_launch must be in 1..64");
     if (cfg.block % 64 || cfg.block > 1024) throw Error(MC_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
@@ -1282,7 +1286,7 @@ void Engine::set_cfg(const mc_engine_cfg& c)
     d->cfg.iters_per_graph = c.iters_per_graph;
     d->cfg.use_graph = c.use_graph;
     d->cfg.drain_divisor = c.drain_divisor == 0 ? 1 : c.drain_divisor;
-    d->cfg.events_per_launch = c.events_per_launch == 0 ? 4 : c.events_per_launch;
+    d->cfg.events_per_launch = c.events_per_launch == 0 ? 8 : c.events_per_launch;
     if (c.bank_capacity != d->cfg.bank_capacity) d->cfg.bank_capacity = c.bank_capacity;
     ++d->generation;
 }
@@ -1290,7 +1294,6 @@ void Engine::set_cfg(const mc_engine_cfg& c)
 mc_engine_cfg Engine::cfg() const { return d->cfg; }
 
 int Engine::n_problems() const { return (int)d->prob.size(); }
-bool Engine::has_tables() const { return d->has_tables; }
 uint64_t Engine::deck_histories(int p) const { return d->probs[p]->n_histories; }
 
 void selftest_math(const double* in, double* out, uint64_t n)

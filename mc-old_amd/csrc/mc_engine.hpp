@@ -24,7 +24,6 @@ public:
     void kernel_stats(mc_kernel_stats* out, bool reset);
     mc_engine_cfg cfg() const;
     int n_problems() const;
-    bool has_tables() const;      // some material is tabulated (continuous energy)
     uint64_t deck_histories(int p) const;
 
 private:

@@ -17,9 +17,10 @@
 //                     conditions, cell search, surface tallies, importance
 //                     split or roulette, weight roulette (general.cpp:89-115,
 //                     population_control.cpp:9-49); collisions go to the
-//                     workgroup's own segment of queue C (one-speed forward
-//                     runs collide right after the flight, in the same thread,
-//                     and launch no k_collision)
+//                     workgroup's own segment of queue C (forward runs collide
+//                     right after the flight, in the same thread, and fly on
+//                     for up to events_per_launch flights; they launch no
+//                     k_collision)
 //   k_collision       per workgroup, the segment its flight workgroup filled:
 //                     collision tallies, implicit capture, nuclide selection,
 //                     elastic scatter, weight roulette (general.cpp:121-163)
@@ -1159,13 +1160,14 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s);
 template <bool kTables, bool kGrad>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
-    // One-speed forward runs collide in the same pass (a collision there is a
-    // few dozen instructions; a kernel of its own would re-read the state),
-    // and a particle that survives its collision flies again, up to
-    // events_per_launch flights, before the bank is refilled.  Every particle
+    // Forward runs collide in the same pass (a kernel of its own would re-read
+    // the state), and a particle that survives its collision flies again, up
+    // to events_per_launch flights (continuous energy: after a lookup of
+    // Sigma_t at its new energy), before the bank is refilled.  Every particle
     // still takes its own events in order; only how many of them one
-    // iteration holds changes.
-    constexpr bool kFuse = !kTables && !kGrad;
+    // iteration holds changes.  Gradient runs keep one flight per iteration
+    // and the collision kernel.
+    constexpr bool kFuse = !kGrad;
     uint32_t extra = 0;                 // flights beyond the first, this thread
     constexpr uint32_t kXq = 2048;      // >= the largest workgroup
     __shared__ uint32_t xq[kXq];
@@ -1192,6 +1194,14 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                 if (!kFuse || st != ST_COLL) break;
                 st = collision_one<kTables, false, false>(A, s);
                 if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
+                if constexpr (kTables) {
+                    // Sigma_t and its grid interval at the energy the collision left
+                    const int m = A.pmat[s];
+                    int g = 0;
+                    const double r = m >= 0 ? lookup_sigt(A, nullptr, m, A.E[s], g) : 0.0;
+                    A.sigt[s] = r;
+                    A.gidx[s] = g;
+                }
             }
             extra += nfl - 1;
         }

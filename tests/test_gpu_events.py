@@ -1,8 +1,8 @@
-"""Flights per particle per event iteration (one-speed forward runs).
+"""Flights per particle per event iteration (forward runs).
 
-The flight kernel of a one-speed forward run collides in the same thread and
-lets a surviving particle fly again, up to events_per_launch flights before
-the bank is refilled.  Each particle still takes its own events in order, so
+The flight kernel of a forward run collides in the same thread and lets a
+surviving particle fly again (continuous energy: after a lookup at its new
+energy), up to events_per_launch flights before the bank is refilled.  Each particle still takes its own events in order, so
 every history stays bit-identical to the oracle's and the flight count is
 exact whatever the setting; only the grouping of events into iterations
 changes.
@@ -10,7 +10,7 @@ changes.
 import numpy as np
 import pytest
 
-from mc_testlib import decks, importance_slab, mc, oracle_run
+from mc_testlib import decks, free_gas_slab, importance_slab, mc, oracle_run, shield_xs_dir
 
 pytestmark = pytest.mark.gpu
 
@@ -36,6 +36,29 @@ def test_slab_flux_bins_any_events_per_launch(k, drain):
     assert g.flights == o.flights
     np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
     np.testing.assert_allclose(g.sq, o.sq, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_shield_any_events_per_launch(k):
+    """Continuous-energy shield: the interval and Sigma_t a further flight
+    uses come from the lookup after the collision, the same values the lookup
+    kernel gives."""
+    n = 4000
+    p = mc.Problem.from_string(decks.shield_deck(n), shield_xs_dir())
+    g = mc.Engine(p, bank_capacity=1 << 12, events_per_launch=k).forward(trace=n)
+    o = oracle_run(p, n, trace=n)
+    _traces_equal(g.trace, o.trace, f"k={k}")
+    assert g.flights == o.flights
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+
+
+def test_free_gas_several_events_per_launch():
+    n = 3000
+    p = mc.Problem.from_string(free_gas_slab(n))
+    g = mc.Engine(p, bank_capacity=1 << 11, events_per_launch=5, drain_divisor=-1).forward(trace=n)
+    o = oracle_run(p, n, trace=n)
+    _traces_equal(g.trace, o.trace, "free gas")
+    assert g.flights == o.flights
 
 
 @pytest.mark.parametrize("k", [1, 8])
