@@ -24,7 +24,7 @@ __global__ void k_refill_count(KArgs A);
 template <bool kGrad> __global__ void k_refill_scatter(KArgs A);
 __global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* c);
 __global__ void k_xs_lookup(KArgs A, int use_lds);
-template <bool kTables, bool kGrad> __global__ void k_flight(KArgs A);
+template <bool kTables, bool kGrad, bool kFuse> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
 template <bool kTables> __global__ void k_cross(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
@@ -761,12 +761,17 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     if (has_tables) hipLaunchKernelGGL(k_xs_lookup, g, b, 0, st, A, 0);
     mark(2, 1);
     mark(3, 0);
+    // every forward run, and gradient runs of several flights per iteration,
+    // collide in the flight kernel
+    const bool fused = !grad || A.events_per_launch > 1;
     if (has_tables) {
-        if (grad) hipLaunchKernelGGL((k_flight<true, true>), g, b, 0, st, A);
-        else hipLaunchKernelGGL((k_flight<true, false>), g, b, 0, st, A);
+        if (!grad) hipLaunchKernelGGL((k_flight<true, false, true>), g, b, 0, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<true, true, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_flight<true, true, false>), g, b, 0, st, A);
     } else {
-        if (grad) hipLaunchKernelGGL((k_flight<false, true>), g, b, 0, st, A);
-        else hipLaunchKernelGGL((k_flight<false, false>), g, b, 0, st, A);
+        if (!grad) hipLaunchKernelGGL((k_flight<false, false, true>), g, b, 0, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<false, true, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_flight<false, true, false>), g, b, 0, st, A);
     }
     // gradient runs cross surfaces in a kernel of their own (timed with the flight)
     if (grad) {
@@ -775,14 +780,9 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     }
     mark(3, 1);
     mark(4, 0);
-    if (!grad) {
-        // forward runs: the flight kernel collided
-    } else if (has_tables) {
-        if (grad) hipLaunchKernelGGL((k_collision<true, true>), g, b, 0, st, A);
-        else hipLaunchKernelGGL((k_collision<true, false>), g, b, 0, st, A);
-    } else {
-        if (grad) hipLaunchKernelGGL((k_collision<false, true>), g, b, 0, st, A);
-        else hipLaunchKernelGGL((k_collision<false, false>), g, b, 0, st, A);
+    if (!fused) {
+        if (has_tables) hipLaunchKernelGGL((k_collision<true, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_collision<false, true>), g, b, 0, st, A);
     }
     mark(4, 1);
     // event counts for the statistics of profiled runs (outside the timed pairs)
@@ -876,8 +876,9 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                 launch_iteration(A, grad, stream, kev.data());
                 HIP_CHECK(hipEventSynchronize(kev[2 * kIterKernels - 1]));
                 for (int j = 0; j < kIterKernels; ++j) {
-                    // no lookup kernel without tables, no collision kernel in forward runs
-                    if ((j == 2 && !has_tables) || (j == 4 && !grad)) continue;
+                    // no lookup kernel without tables, no collision kernel when the
+                    // flight kernel collides
+                    if ((j == 2 && !has_tables) || (j == 4 && (!grad || A.events_per_launch > 1))) continue;
                     float ms = 0.f;
                     HIP_CHECK(hipEventElapsedTime(&ms, kev[2 * j], kev[2 * j + 1]));
                     kstats.seconds[j] += 1e-3 * ms;

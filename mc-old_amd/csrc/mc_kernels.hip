@@ -16,12 +16,13 @@
 //                     by the workgroup (gradient runs: by k_cross): boundary
 //                     conditions, cell search, surface tallies, importance
 //                     split or roulette, weight roulette (general.cpp:89-115,
-//                     population_control.cpp:9-49); collisions go to the
-//                     workgroup's own segment of queue C (forward runs collide
-//                     right after the flight, in the same thread, and fly on
-//                     for up to events_per_launch flights; they launch no
-//                     k_collision)
-//   k_collision       per workgroup, the segment its flight workgroup filled:
+//                     population_control.cpp:9-49); a flight's collision
+//                     follows in the same thread and a surviving particle
+//                     flies on, up to events_per_launch flights per launch
+//                     (gradient runs at one flight per launch instead queue
+//                     collisions in the workgroup's own segment of queue C)
+//   k_collision       (gradient runs at one flight per launch) per workgroup,
+//                     the segment its flight workgroup filled:
 //                     collision tallies, implicit capture, nuclide selection,
 //                     elastic scatter, weight roulette (general.cpp:121-163)
 // k_drain carries the last particles of a run from event to event through the
@@ -1157,17 +1158,16 @@ __device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
 template <bool kTables, bool kGrad, bool kInlineFreeGas = true>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s);
 
-template <bool kTables, bool kGrad>
+template <bool kTables, bool kGrad, bool kFuse>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
-    // Forward runs collide in the same pass (a kernel of its own would re-read
-    // the state), and a particle that survives its collision flies again, up
-    // to events_per_launch flights (continuous energy: after a lookup of
-    // Sigma_t at its new energy), before the bank is refilled.  Every particle
-    // still takes its own events in order; only how many of them one
-    // iteration holds changes.  Gradient runs keep one flight per iteration
-    // and the collision kernel.
-    constexpr bool kFuse = !kGrad;
+    // kFuse: the flight's collision happens in the same pass (a kernel of its
+    // own would re-read the state), and a particle that survives it flies
+    // again, up to events_per_launch flights (continuous energy: after a
+    // lookup of Sigma_t at its new energy), before the bank is refilled.
+    // Every particle still takes its own events in order; only how many of
+    // them one iteration holds changes.  Without kFuse (gradient runs at one
+    // flight per iteration) collisions go to queue C for k_collision.
     uint32_t extra = 0;                 // flights beyond the first, this thread
     constexpr uint32_t kXq = 2048;      // >= the largest workgroup
     __shared__ uint32_t xq[kXq];
@@ -1192,7 +1192,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                 st = flight_one<kTables, kGrad, true>(A, s, crossed);
                 ++nfl;
                 if (!kFuse || st != ST_COLL) break;
-                st = collision_one<kTables, false, false>(A, s);
+                st = collision_one<kTables, kGrad, false>(A, s);
                 if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
                 if constexpr (kTables) {
                     // Sigma_t and its grid interval at the energy the collision left
@@ -1669,18 +1669,18 @@ __global__ void k_selftest_math(const double* in, double* out, uint64_t n)
 // explicit instantiations used by the host
 template __global__ void k_refill_scatter<false>(KArgs);
 template __global__ void k_refill_scatter<true>(KArgs);
-template __global__ void k_flight<false, false>(KArgs);
-template __global__ void k_flight<true, false>(KArgs);
-template __global__ void k_flight<false, true>(KArgs);
-template __global__ void k_flight<true, true>(KArgs);
 template __global__ void k_drain<false, false>(KArgs, uint32_t);
 template __global__ void k_drain<true, false>(KArgs, uint32_t);
 template __global__ void k_drain<false, true>(KArgs, uint32_t);
 template __global__ void k_drain<true, true>(KArgs, uint32_t);
-template __global__ void k_collision<false, false>(KArgs);
-template __global__ void k_collision<true, false>(KArgs);
 template __global__ void k_collision<false, true>(KArgs);
 template __global__ void k_collision<true, true>(KArgs);
+template __global__ void k_flight<false, false, true>(KArgs);
+template __global__ void k_flight<true, false, true>(KArgs);
+template __global__ void k_flight<false, true, false>(KArgs);
+template __global__ void k_flight<true, true, false>(KArgs);
+template __global__ void k_flight<false, true, true>(KArgs);
+template __global__ void k_flight<true, true, true>(KArgs);
 template __global__ void k_cross<false>(KArgs);
 template __global__ void k_cross<true>(KArgs);
 

@@ -96,6 +96,24 @@ def test_splitting_with_several_events_per_launch():
     np.testing.assert_allclose(g.sq, o.sq, rtol=1e-12, atol=1e-300)
 
 
+@pytest.mark.parametrize("k", [1, 8])
+def test_gradient_any_events_per_launch(k):
+    """Gradient runs: at one flight per iteration the collision kernel takes
+    the collisions; at several the flight kernel folds them in the same pass.
+    The likelihood-ratio sums equal the oracle's either way."""
+    n = 4000
+    p = mc.Problem.from_string(decks.shield_deck(n, params="24"), shield_xs_dir())
+    eng = mc.Engine(p, bank_capacity=1 << 12, events_per_launch=k)
+    rng = np.random.default_rng(4)
+    go = rng.normal(size=p.n_bins)
+    theta = rng.uniform(0.9, 1.1, p.n_params)
+    g = eng.backward(go, theta=theta)
+    o = oracle_run(p, n, theta=theta, grad_out=go)
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(g.grad, o.grad, rtol=1e-9, atol=1e-13)
+    assert g.flights == o.flights
+
+
 def test_events_per_launch_out_of_range():
     p = mc.Problem.from_string(decks.slab_deck(1.0, 0.5, 0, 100))
     with pytest.raises(mc.MCError) as e:
