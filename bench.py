@@ -48,11 +48,13 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD public spec
 # and not counted.
 ALGO_BYTES = {
     "k_xs_lookup": 28,       # queue 4 + material 4 + E 8 in; Sigma_t 8 + grid interval 4 out
-    "k_flight": 189,         # forward runs, per particle per launch (up to events_per_launch flights,
-                             # each with its collision, the state kept between them):
-                             # queue 4, cell 4, pos 24, dir 24, rng 8, E 8, Sigma_t 8, interval 4, prob 4,
-                             # count 4, wgt 8 in; pos 24, dir 24, E 8, wgt 8, rng 8, count 4, status 1,
-                             # Sigma_t 8, interval 4 out (a surface crossing adds cell, material)
+    "k_flight": 278,         # forward runs, per flight with its collision (a particle takes up to
+                             # events_per_launch of them per launch; each event reads and writes the
+                             # state it uses): flight 137 (queue 4, cell 4, pos 24, dir 24, rng 8, E 8,
+                             # Sigma_t 8, prob 4, count 4, wgt 8 in; pos 24, rng 8, count 4, status 1,
+                             # queue 4 out) + collision 113 (cell 4, prob 4, wgt 8, rng 8, E 8, dir 24,
+                             # interval 4 in; dir 24, E 8, wgt 8, rng 8, status 1 out) + the lookup at
+                             # the new energy 28 (material 4, E 8 in; Sigma_t 8, interval 4 out, queue 4)
     "k_collision": 113,      # gradient runs: queue 4, cell 4, prob 4, wgt 8, rng 8, E 8, dir 24, interval 4 in;
                              # dir 24, E 8, wgt 8, rng 8, status 1 out
     "k_refill_count": 1,     # status byte per slot

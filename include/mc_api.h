@@ -198,8 +198,8 @@ typedef struct mc_engine_cfg {
 /* Per-kernel device time of the event loop, recorded when cfg.profile = 1:
  * kernels in order refill_count, refill_scatter, xs_lookup, flight (with the
  * surface crossings; in forward runs also the collisions and the further
- * flights of each particle), collision (gradient runs), drain.  Items of the
- * flight kernel are particles per launch, not flights.                      */
+ * flights of each particle), collision (gradient runs at one flight per
+ * iteration), drain.  Items of the flight kernel are flights.               */
 #define MC_N_KERNELS 6
 typedef struct mc_kernel_stats {
     uint64_t launches[MC_N_KERNELS];

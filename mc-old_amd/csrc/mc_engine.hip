@@ -898,13 +898,13 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                                          " s (MC_MAX_SECONDS); live particles: " + std::to_string(h_ctr->nA));
     }
     if (profile) {
-        // event counts: the lookup and flight kernels saw every live particle
-        // of the iterations (forward flights beyond a particle's first in a
-        // launch are not items of their own), then collisions; the drain's
-        // flights are its own
-        const uint64_t it_particles = h_ctr->tA - drain_flights - h_ctr->xfl;
-        kstats.items[2] += has_tables ? it_particles : 0;
-        kstats.items[3] += it_particles;
+        // event counts: the lookup kernel saw every live particle of the
+        // iterations, the flight kernel took their flights (several per
+        // particle when it collides and flies on), then collisions; the
+        // drain's flights are its own
+        const uint64_t it_flights = h_ctr->tA - drain_flights;
+        kstats.items[2] += has_tables ? it_flights - h_ctr->xfl : 0;
+        kstats.items[3] += it_flights;
         kstats.items[4] += h_ctr->tC - drain_coll;
         kstats.items[kDrainSlot] += drain_flights;
     }
