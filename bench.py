@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--xs-only", action="store_true", help="only the standalone cross-section lookup")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="headline and its per-kernel roofline only (for profiling the headline kernels)")
     ap.add_argument("--cpu-worker", nargs=3, metavar=("OFFSET", "N", "XS"), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
@@ -237,7 +239,7 @@ def main():
             log(f"traffic passes skipped: {ex}")
 
     cpu_par = None
-    if not args.quick:
+    if not (args.quick or args.roofline_only):
         # child processes too, so also before this process uses the GPU
         try:
             cpu_par = cpu_parallel(shield_xs_dir(), max(1000, args.cpu_sample // 2))
@@ -340,6 +342,9 @@ def main():
         out["kernels"] = table
         log("kernels:", json.dumps(table))
 
+        if args.roofline_only:
+            print(json.dumps(out), flush=True)
+            return
         # ------------------------------------------ standalone XS lookup
         out["xs_lookup"] = xs_lookup_bench(mc, decks, xs)
 
