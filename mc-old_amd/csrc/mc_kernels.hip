@@ -1067,13 +1067,56 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
 // it here.  Returns the new status (ST_COLL: a collision is due).
 // kDefer: a flight ending on a surface stores the particle and the surface
 // and returns ST_SURF; the crossing is left to cross_deferred.
-template <bool kTables, bool kGrad, bool kDefer>
-__device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& crossed)
+// State of a particle held in registers across the events of one launch
+// (forward runs): kReg routines read and update it instead of the bank, and
+// the caller stores it once when the particle stops.
+struct PState {
+    Vec3 pos, dir;
+    double E, w, sigt;
+    uint64_t rng;
+    int cell, prob, gidx;
+    uint32_t nfl;
+};
+
+__device__ __forceinline__ PState load_state(const KArgs& A, uint32_t s, bool tables)
 {
-    const int c = A.pcell[s];
+    PState P;
+    P.cell = A.pcell[s];
+    P.prob = A.pprob[s];
+    P.pos = {A.x[s], A.y[s], A.z[s]};
+    P.dir = {A.u[s], A.v[s], A.w[s]};
+    P.E = A.E[s];
+    P.w = A.wgt[s];
+    P.rng = A.rng[s];
+    P.nfl = A.nflight[s];
+    P.sigt = tables ? A.sigt[s] : 0.0;
+    P.gidx = tables ? A.gidx[s] : 0;
+    return P;
+}
+
+// The fields an event can change (the cell changes only at a crossing, which
+// stores its own).
+__device__ __forceinline__ void store_state(const KArgs& A, uint32_t s, const PState& P, bool tables)
+{
+    A.x[s] = P.pos.x; A.y[s] = P.pos.y; A.z[s] = P.pos.z;
+    A.u[s] = P.dir.x; A.v[s] = P.dir.y; A.w[s] = P.dir.z;
+    A.E[s] = P.E;
+    A.wgt[s] = P.w;
+    A.rng[s] = P.rng;
+    A.nflight[s] = P.nfl;
+    if (tables) {
+        A.sigt[s] = P.sigt;
+        A.gidx[s] = P.gidx;
+    }
+}
+
+template <bool kTables, bool kGrad, bool kDefer, bool kReg = false>
+__device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& crossed, PState* R = nullptr)
+{
+    const int c = kReg ? R->cell : A.pcell[s];
     const mc_cell_desc C = A.cell[c];
-    Vec3 p = {A.x[s], A.y[s], A.z[s]};
-    const Vec3 u = {A.u[s], A.v[s], A.w[s]};
+    Vec3 p = kReg ? R->pos : Vec3{A.x[s], A.y[s], A.z[s]};
+    const Vec3 u = kReg ? R->dir : Vec3{A.u[s], A.v[s], A.w[s]};
     // surface_intersect, general.cpp:54-67
     double dist = kMaxFloat;
     int S = -1;
@@ -1084,19 +1127,19 @@ __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& 
         if (d < dist) { dist = d; S = sf; }
     }
     // collision_distance, general.cpp:40-48
-    uint64_t rng = A.rng[s];
-    const double E = A.E[s];
+    uint64_t rng = kReg ? R->rng : A.rng[s];
+    const double E = kReg ? R->E : A.E[s];
     double dcol, sigt = 0.0;
     if (C.material >= 0) {
         // without tables every material is all-constant: its per-run Sigma_t
-        sigt = kTables ? A.sigt[s] : A.mat_tconst[C.material];
+        sigt = kTables ? (kReg ? R->sigt : A.sigt[s]) : A.mat_tconst[C.material];
         dcol = -mc_log(urand(rng)) / sigt;
     } else {
         dcol = kMaxFloatLess;
     }
     const bool hit = dcol > dist;
     const double l = hit ? dist : dcol;
-    const int p_id = A.pprob[s];
+    const int p_id = kReg ? R->prob : A.pprob[s];
     const DProb PR = A.prob[p_id];
     // gradient runs: the flight's -l dSigma_t/dtheta is folded into S at the
     // collision or surface crossing that ends it, where the per-nuclide
@@ -1106,23 +1149,32 @@ __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& 
     p.x += u.x * l;
     p.y += u.y * l;
     p.z += u.z * l;
-    const uint32_t nfl = A.nflight[s] + 1;
-    const double w = A.wgt[s];
+    const uint32_t nfl = (kReg ? R->nfl : A.nflight[s]) + 1;
+    const double w = kReg ? R->w : A.wgt[s];
     for (int a = 0; a < C.tl_count; ++a)
         score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, l, sigt, !hit);
     uint8_t st;
     crossed = false;
+    if (kReg) {
+        R->pos = p;
+        R->rng = rng;
+        R->nfl = nfl;
+    }
     if (!hit) {
-        A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-        A.rng[s] = rng;
-        A.nflight[s] = nfl;
+        if (!kReg) {
+            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+            A.rng[s] = rng;
+            A.nflight[s] = nfl;
+        }
         st = ST_COLL;
     } else if (S >= 0) {
         crossed = true;
         if (kDefer) {
-            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-            A.rng[s] = rng;
-            A.nflight[s] = nfl;
+            if (!kReg) {
+                A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+                A.rng[s] = rng;
+                A.nflight[s] = nfl;
+            }
             A.psurf[s] = S;
             st = ST_SURF;
         } else {
@@ -1130,13 +1182,17 @@ __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& 
         }
     } else {                        // no surface ahead: lost particle
         atomicAdd(&A.ctr->err_lost, 1u);
-        A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
-        A.rng[s] = rng;
-        A.nflight[s] = nfl;
-        A.wgt[s] = 0.0;
+        if (kReg) {
+            R->w = 0.0;
+        } else {
+            A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
+            A.rng[s] = rng;
+            A.nflight[s] = nfl;
+            A.wgt[s] = 0.0;
+        }
         st = finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, nfl);
     }
-    A.status[s] = st;
+    if (!kReg) A.status[s] = st;
     return st;
 }
 
@@ -1155,8 +1211,9 @@ __device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
 // matter, every crossing is independent) and crossed by the whole block
 // once the queue fills or the range ends, so the rare crossing does not hold
 // a wave's other lanes and its registers stay out of the flight loop.
-template <bool kTables, bool kGrad, bool kInlineFreeGas = true>
-__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s);
+struct PState;
+template <bool kTables, bool kGrad, bool kInlineFreeGas = true, bool kReg = false>
+__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg = nullptr);
 
 template <bool kTables, bool kGrad, bool kFuse>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
@@ -1188,19 +1245,40 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         if (i < hi) {
             bool crossed;
             uint32_t nfl = 0;
-            for (;;) {
-                st = flight_one<kTables, kGrad, true>(A, s, crossed);
-                ++nfl;
-                if (!kFuse || st != ST_COLL) break;
-                st = collision_one<kTables, kGrad, false>(A, s);
-                if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
-                if constexpr (kTables) {
-                    // Sigma_t and its grid interval at the energy the collision left
-                    const int m = A.pmat[s];
-                    int g = 0;
-                    const double r = m >= 0 ? lookup_sigt(A, nullptr, m, A.E[s], g) : 0.0;
-                    A.sigt[s] = r;
-                    A.gidx[s] = g;
+            if constexpr (kFuse && !kGrad) {
+                // forward: the state stays in registers from event to event
+                PState R = load_state(A, s, kTables);
+                const int m = kTables ? A.pmat[s] : -1;
+                for (;;) {
+                    st = flight_one<kTables, false, true, true>(A, s, crossed, &R);
+                    ++nfl;
+                    if (st != ST_COLL) break;
+                    st = collision_one<kTables, false, false, true>(A, s, &R);
+                    if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
+                    if constexpr (kTables) {
+                        // Sigma_t and its grid interval at the energy the collision left
+                        int g = 0;
+                        R.sigt = m >= 0 ? lookup_sigt(A, nullptr, m, R.E, g) : 0.0;
+                        R.gidx = g;
+                    }
+                }
+                store_state(A, s, R, kTables);
+                A.status[s] = st;
+            } else {
+                for (;;) {
+                    st = flight_one<kTables, kGrad, true>(A, s, crossed);
+                    ++nfl;
+                    if (!kFuse || st != ST_COLL) break;
+                    st = collision_one<kTables, kGrad, false>(A, s);
+                    if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
+                    if constexpr (kTables) {
+                        // Sigma_t and its grid interval at the energy the collision left
+                        const int m = A.pmat[s];
+                        int g = 0;
+                        const double r = m >= 0 ? lookup_sigt(A, nullptr, m, A.E[s], g) : 0.0;
+                        A.sigt[s] = r;
+                        A.gidx[s] = g;
+                    }
                 }
             }
             extra += nfl - 1;
@@ -1299,18 +1377,20 @@ __device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
     return f;
 }
 
-template <bool kTables, bool kGrad, bool kInlineFreeGas>
-__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
+// kReg: the particle's state is *reg (forward runs), read and updated there; the
+// returned status is not stored (the caller stores the state and the status).
+template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg>
+__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg)
 {
-    const int c = A.pcell[s];
+    const int c = kReg ? reg->cell : A.pcell[s];
     const mc_cell_desc C = A.cell[c];
-    const int p_id = A.pprob[s];
+    const int p_id = kReg ? reg->prob : A.pprob[s];
     const DProb PR = A.prob[p_id];
-    double w = A.wgt[s];
-    uint64_t rng = A.rng[s];
-    double E = A.E[s];
+    double w = kReg ? reg->w : A.wgt[s];
+    uint64_t rng = kReg ? reg->rng : A.rng[s];
+    double E = kReg ? reg->E : A.E[s];
     bool alive = true;
-    Vec3 u = {A.u[s], A.v[s], A.w[s]};
+    Vec3 u = kReg ? reg->dir : Vec3{A.u[s], A.v[s], A.w[s]};
     if (C.material < 0) {                    // void: kill at collision
         alive = false;
         w = 0.0;
@@ -1324,7 +1404,13 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
         const mc_material_desc M = A.mat[m];
         const bool scaled = M.param_count != 0;
         XsPoint P;
-        const int gm = kTables ? material_point_cached(A, m, s, E, P) : material_point(A, m, E, P);
+        int gm;
+        if constexpr (kTables && kReg) {
+            gm = A.mat_grid[m];
+            if (gm >= 0) xs_point_at(A, gm, reg->gidx, E, P);
+        } else {
+            gm = kTables ? material_point_cached(A, m, s, E, P) : material_point(A, m, E, P);
+        }
         // Sigma_s, Sigma_c (and Sigma_t when no lookup kernel ran) in one pass.
         // The first kNucReg entries keep in registers what later steps reuse:
         // the products sigma_s * N for the nuclide choice (the same terms
@@ -1369,7 +1455,7 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
         }
         // Sigma_t: the lookup kernel's value for this particle is the same sum
         if (scaled) X.t = X.s + X.c;
-        else if (kTables) X.t = A.sigt[s];
+        else if (kTables) X.t = kReg ? reg->sigt : A.sigt[s];
         // implicit capture, general.cpp:154-156 (Sigma_f = 0)
         const double implicit = X.c + 0.0;
         w = w * (X.t - implicit) / X.t;
@@ -1476,18 +1562,27 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s)
         }
     }
     weight_roulette(w, alive, rng, PR);
-    A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
-    A.E[s] = E;
-    A.wgt[s] = w;
-    A.rng[s] = rng;
-    if (alive) {
-        A.status[s] = ST_ALIVE;
-        return ST_ALIVE;
+    if constexpr (kReg) {
+        reg->dir = u;
+        reg->E = E;
+        reg->w = w;
+        reg->rng = rng;
+        if (alive) return ST_ALIVE;
+        return finish_history<kGrad, false>(A, s, p_id, PR, rng, reg->pos, E, reg->nfl);
     } else {
-        const Vec3 p = {A.x[s], A.y[s], A.z[s]};
-        A.status[s] = finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
+        A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
+        A.E[s] = E;
+        A.wgt[s] = w;
+        A.rng[s] = rng;
+        if (alive) {
+            A.status[s] = ST_ALIVE;
+            return ST_ALIVE;
+        } else {
+            const Vec3 p = {A.x[s], A.y[s], A.z[s]};
+            A.status[s] = finish_history<kGrad, false>(A, s, p_id, PR, rng, p, E, A.nflight[s]);
+        }
+        return ST_DEAD;
     }
-    return ST_DEAD;
 }
 
 template <bool kTables, bool kGrad>
