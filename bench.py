@@ -48,13 +48,12 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD public spec
 # and not counted.
 ALGO_BYTES = {
     "k_xs_lookup": 28,       # queue 4 + material 4 + E 8 in; Sigma_t 8 + grid interval 4 out
-    "k_flight": 278,         # forward runs, per flight with its collision (a particle takes up to
-                             # events_per_launch of them per launch; each event reads and writes the
-                             # state it uses): flight 137 (queue 4, cell 4, pos 24, dir 24, rng 8, E 8,
-                             # Sigma_t 8, prob 4, count 4, wgt 8 in; pos 24, rng 8, count 4, status 1,
-                             # queue 4 out) + collision 113 (cell 4, prob 4, wgt 8, rng 8, E 8, dir 24,
-                             # interval 4 in; dir 24, E 8, wgt 8, rng 8, status 1 out) + the lookup at
-                             # the new energy 28 (material 4, E 8 in; Sigma_t 8, interval 4 out, queue 4)
+    "k_flight": 189,         # forward runs, per PARTICLE per launch: the particle's state is read once,
+                             # kept in registers through up to events_per_launch flights with their
+                             # collisions and lookups, and written once: queue 4, cell 4, prob 4,
+                             # material 4, pos 24, dir 24, E 8, wgt 8, rng 8, count 4, Sigma_t 8,
+                             # interval 4 in; pos 24, dir 24, E 8, wgt 8, rng 8, count 4, Sigma_t 8,
+                             # interval 4, status 1 out
     "k_collision": 113,      # gradient runs: queue 4, cell 4, prob 4, wgt 8, rng 8, E 8, dir 24, interval 4 in;
                              # dir 24, E 8, wgt 8, rng 8, status 1 out
     "k_refill_count": 1,     # status byte per slot
@@ -316,12 +315,18 @@ def main():
             if s["launches"] == 0 or s["seconds"] == 0:
                 continue
             b = ALGO_BYTES.get(k)
-            gbs = b * s["items"] / s["seconds"] / 1e9 if b else None
+            # the flight kernel's items are flights; its bytes are per particle, and the
+            # particles it took are the lookup kernel's items (one lookup per particle)
+            units = ks["k_xs_lookup"]["items"] if k == "k_flight" else s["items"]
+            gbs = b * units / s["seconds"] / 1e9 if b else None
             table[k] = {"seconds": s["seconds"], "launches": s["launches"], "items": s["items"],
                         "avg_us": 1e6 * s["seconds"] / s["launches"], "algo_GBs": gbs}
+            if k == "k_flight":
+                table[k]["particles"] = units
         dom = max((k for k in table if k in ALGO_BYTES), key=lambda k: table[k]["seconds"])
         d = table[dom]
         epl = d["items"] / d["launches"]
+        upl = d.get("particles", d["items"]) / d["launches"]     # units the algorithmic bytes count
         tr = None
         if traffic and dom in traffic["bytes_per_item"]:
             t = traffic["bytes_per_item"][dom]
@@ -331,8 +336,9 @@ def main():
                     row["hbm_B_per_item"] = traffic["bytes_per_item"][k]
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["algo_GBs"], "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": tr,
-                           "bytes_per_event": ALGO_BYTES[dom], "avg_launch_us": d["avg_us"],
-                           "events_per_launch": epl}
+                           "bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": upl,
+                           "unit_of_bytes": "particle" if dom == "k_flight" else "event",
+                           "avg_launch_us": d["avg_us"], "events_per_launch": epl}
         if traffic:
             out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE / WRITE_SIZE over the traffic probe "
                                                  f"({traffic['histories']} histories, plain launches), scaled by a "

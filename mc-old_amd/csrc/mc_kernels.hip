@@ -1245,15 +1245,15 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
         if (i < hi) {
             bool crossed;
             uint32_t nfl = 0;
-            if constexpr (kFuse && !kGrad) {
-                // forward: the state stays in registers from event to event
+            if constexpr (kFuse) {
+                // the state stays in registers from event to event
                 PState R = load_state(A, s, kTables);
                 const int m = kTables ? A.pmat[s] : -1;
                 for (;;) {
-                    st = flight_one<kTables, false, true, true>(A, s, crossed, &R);
+                    st = flight_one<kTables, kGrad, true, true>(A, s, crossed, &R);
                     ++nfl;
                     if (st != ST_COLL) break;
-                    st = collision_one<kTables, false, false, true>(A, s, &R);
+                    st = collision_one<kTables, kGrad, false, true>(A, s, &R);
                     if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
                     if constexpr (kTables) {
                         // Sigma_t and its grid interval at the energy the collision left
