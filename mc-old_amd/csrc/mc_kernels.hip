@@ -1067,9 +1067,9 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
 // it here.  Returns the new status (ST_COLL: a collision is due).
 // kDefer: a flight ending on a surface stores the particle and the surface
 // and returns ST_SURF; the crossing is left to cross_deferred.
-// State of a particle held in registers across the events of one launch
-// (forward runs): kReg routines read and update it instead of the bank, and
-// the caller stores it once when the particle stops.
+// State of a particle held in registers across the events of one launch of
+// the flight kernel: kReg routines read and update it instead of the bank,
+// and the caller stores it once when the particle stops.
 struct PState {
     Vec3 pos, dir;
     double E, w, sigt;
@@ -1265,21 +1265,9 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                 store_state(A, s, R, kTables);
                 A.status[s] = st;
             } else {
-                for (;;) {
-                    st = flight_one<kTables, kGrad, true>(A, s, crossed);
-                    ++nfl;
-                    if (!kFuse || st != ST_COLL) break;
-                    st = collision_one<kTables, kGrad, false>(A, s);
-                    if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
-                    if constexpr (kTables) {
-                        // Sigma_t and its grid interval at the energy the collision left
-                        const int m = A.pmat[s];
-                        int g = 0;
-                        const double r = m >= 0 ? lookup_sigt(A, nullptr, m, A.E[s], g) : 0.0;
-                        A.sigt[s] = r;
-                        A.gidx[s] = g;
-                    }
-                }
+                // one flight; its collision goes to queue C
+                st = flight_one<kTables, kGrad, true>(A, s, crossed);
+                nfl = 1;
             }
             extra += nfl - 1;
         }
@@ -1377,8 +1365,8 @@ __device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
     return f;
 }
 
-// kReg: the particle's state is *reg (forward runs), read and updated there; the
-// returned status is not stored (the caller stores the state and the status).
+// kReg: the particle's state is *reg, read and updated there; the returned
+// status is not stored (the caller stores the state and the status).
 template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg)
 {
