@@ -191,7 +191,7 @@ typedef struct mc_engine_cfg {
                                 in one drain launch (0: 1, < 0: never)        */
     int32_t events_per_launch; /* forward runs: flights, each with its
                                 collision, a particle takes per event iteration
-                                before the bank is refilled (0: 8; 1: one);
+                                before the bank is refilled (0: 16; 1: one);
                                 runs that can split take one                  */
 } mc_engine_cfg;
 

@@ -255,7 +255,7 @@ mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* o
         const uint64_t banks[] = {1ull << 19, 1ull << 20, 1ull << 21};
         const int drains[] = {1, -1};
         // flights per particle per iteration (forward runs)
-        const std::vector<int> events = {1, 2, 4, 6, 8};
+        const std::vector<int> events = {1, 2, 4, 6, 8, 16};
         for (uint64_t B : banks) {
             if (B > 4 * sampled && B != banks[0]) continue;
             for (int blk : blocks)
