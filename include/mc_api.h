@@ -188,7 +188,7 @@ typedef struct mc_engine_cfg {
     int32_t profile;         /* 1: plain launches timed per kernel (mc_engine_kernel_stats) */
     int32_t drain_divisor;   /* once every history has started and at most
                                 bank/drain_divisor particles live, finish them
-                                in one drain launch (0: 1, < 0: never)        */
+                                in one drain launch (0: 4, < 0: never)        */
     int32_t events_per_launch; /* forward runs: flights, each with its
                                 collision, a particle takes per event iteration
                                 before the bank is refilled (0: 16; 1: one);

@@ -1146,7 +1146,7 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     if (cfg.grid_per_cu == 0) cfg.grid_per_cu = 4;
     if (cfg.iters_per_graph == 0) cfg.iters_per_graph = 16;
     if (cfg.bank_capacity == 0) cfg.bank_capacity = 1ull << 21;
-    if (cfg.drain_divisor == 0) cfg.drain_divisor = 1;
+    if (cfg.drain_divisor == 0) cfg.drain_divisor = 4;
     if (cfg.events_per_launch == 0) cfg.events_per_launch = 16;
     if (cfg.events_per_launch < 1 || cfg.events_per_launch > 64)
         throw Error(MC_ERR_ARGUMENT, "events_per_launch must be in 1..64");
@@ -1286,7 +1286,7 @@ void Engine::set_cfg(const mc_engine_cfg& c)
     d->cfg.grid_per_cu = c.grid_per_cu;
     d->cfg.iters_per_graph = c.iters_per_graph;
     d->cfg.use_graph = c.use_graph;
-    d->cfg.drain_divisor = c.drain_divisor == 0 ? 1 : c.drain_divisor;
+    d->cfg.drain_divisor = c.drain_divisor == 0 ? 4 : c.drain_divisor;
     d->cfg.events_per_launch = c.events_per_launch == 0 ? 16 : c.events_per_launch;
     if (c.bank_capacity != d->cfg.bank_capacity) d->cfg.bank_capacity = c.bank_capacity;
     ++d->generation;
