@@ -196,6 +196,52 @@ def xs_lookup_bench(mc, decks, xs, n_xs=1 << 26):
             "algo_bytes_per_lookup": 20, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
 
+def kernel_roofline(mc, prob, H, traffic):
+    """Per-kernel device time of the headline deck (HIP events around each
+    kernel, a profile run of H/10 histories) and the roofline line of the
+    dominant kernel; returns (roofline, per-kernel table)."""
+    peng = mc.Engine(prob, profile=True)
+    nprof = max(2, H // 10)
+    peng.forward(n_histories=nprof)
+    ks = peng.kernel_stats()
+    table = {}
+    for k, s in ks.items():
+        if s["launches"] == 0 or s["seconds"] == 0:
+            continue
+        b = ALGO_BYTES.get(k)
+        # the flight kernel's items are flights; its bytes are per particle, and the
+        # particles it took are the lookup kernel's items (one lookup per particle)
+        units = ks["k_xs_lookup"]["items"] if k == "k_flight" else s["items"]
+        gbs = b * units / s["seconds"] / 1e9 if b else None
+        table[k] = {"seconds": s["seconds"], "launches": s["launches"], "items": s["items"],
+                    "avg_us": 1e6 * s["seconds"] / s["launches"], "algo_GBs": gbs}
+        if k == "k_flight":
+            table[k]["particles"] = units
+    dom = max((k for k in table if k in ALGO_BYTES), key=lambda k: table[k]["seconds"])
+    d = table[dom]
+    epl = d["items"] / d["launches"]
+    upl = d.get("particles", d["items"]) / d["launches"]     # units the algorithmic bytes count
+    tr = None
+    if traffic and dom in traffic["bytes_per_item"]:
+        t = traffic["bytes_per_item"][dom]
+        tr = (t["fetch_B"] + t["write_B"]) * epl
+        for k, row in table.items():
+            if k in traffic["bytes_per_item"]:
+                row["hbm_B_per_item"] = traffic["bytes_per_item"][k]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": d["algo_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": tr,
+            "bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": upl,
+            "unit_of_bytes": "particle" if dom == "k_flight" else "event",
+            "avg_launch_us": d["avg_us"], "events_per_launch": epl}
+    if traffic:
+        roof["traffic_source"] = ("rocprofv3 FETCH_SIZE / WRITE_SIZE over the traffic probe "
+                                  f"({traffic['histories']} histories, plain launches), scaled by a "
+                                  "512 MiB copy: fetch x{:.3f}, write x{:.3f}".format(
+                                      traffic["copy_calibration"]["fetch_factor"],
+                                      traffic["copy_calibration"]["write_factor"]))
+    return roof, table
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +256,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--xs-only", action="store_true", help="only the standalone cross-section lookup")
     ap.add_argument("--roofline-only", action="store_true",
-                    help="headline and its per-kernel roofline only (for profiling the headline kernels)")
+                    help="only the headline deck's per-kernel profile run and its roofline line "
+                         "(for profiling the dominant kernel)")
     ap.add_argument("--cpu-worker", nargs=3, metavar=("OFFSET", "N", "XS"), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
@@ -254,6 +301,12 @@ def main():
     # ---------------------------------------------------------- headline
     prob = mc.Problem.from_string(decks.shield_deck(H), xs)
     G = not args.no_graph
+    if args.roofline_only:
+        # only the per-kernel profile run, so that a profiler wrapped around
+        # this process sees the dispatches the roofline line is computed from
+        roof, table = kernel_roofline(mc, prob, H, None)
+        print(json.dumps({"roofline": roof, "kernels": table}), flush=True)
+        return
     eng = mc.Engine(prob, use_graph=G)
     for _ in range(args.warmup):
         eng.forward(n_histories=H)
@@ -306,51 +359,8 @@ def main():
 
     if not args.quick:
         # -------------------------------------------- per-kernel roofline
-        peng = mc.Engine(prob, profile=True)
-        nprof = max(2, H // 10)
-        peng.forward(n_histories=nprof)
-        ks = peng.kernel_stats()
-        table = {}
-        for k, s in ks.items():
-            if s["launches"] == 0 or s["seconds"] == 0:
-                continue
-            b = ALGO_BYTES.get(k)
-            # the flight kernel's items are flights; its bytes are per particle, and the
-            # particles it took are the lookup kernel's items (one lookup per particle)
-            units = ks["k_xs_lookup"]["items"] if k == "k_flight" else s["items"]
-            gbs = b * units / s["seconds"] / 1e9 if b else None
-            table[k] = {"seconds": s["seconds"], "launches": s["launches"], "items": s["items"],
-                        "avg_us": 1e6 * s["seconds"] / s["launches"], "algo_GBs": gbs}
-            if k == "k_flight":
-                table[k]["particles"] = units
-        dom = max((k for k in table if k in ALGO_BYTES), key=lambda k: table[k]["seconds"])
-        d = table[dom]
-        epl = d["items"] / d["launches"]
-        upl = d.get("particles", d["items"]) / d["launches"]     # units the algorithmic bytes count
-        tr = None
-        if traffic and dom in traffic["bytes_per_item"]:
-            t = traffic["bytes_per_item"][dom]
-            tr = (t["fetch_B"] + t["write_B"]) * epl
-            for k, row in table.items():
-                if k in traffic["bytes_per_item"]:
-                    row["hbm_B_per_item"] = traffic["bytes_per_item"][k]
-        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": d["algo_GBs"], "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": tr,
-                           "bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": upl,
-                           "unit_of_bytes": "particle" if dom == "k_flight" else "event",
-                           "avg_launch_us": d["avg_us"], "events_per_launch": epl}
-        if traffic:
-            out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE / WRITE_SIZE over the traffic probe "
-                                                 f"({traffic['histories']} histories, plain launches), scaled by a "
-                                                 "512 MiB copy: fetch x{:.3f}, write x{:.3f}".format(
-                                                     traffic["copy_calibration"]["fetch_factor"],
-                                                     traffic["copy_calibration"]["write_factor"]))
-        out["kernels"] = table
-        log("kernels:", json.dumps(table))
-
-        if args.roofline_only:
-            print(json.dumps(out), flush=True)
-            return
+        out["roofline"], out["kernels"] = kernel_roofline(mc, prob, H, traffic)
+        log("kernels:", json.dumps(out["kernels"]))
         # ------------------------------------------ standalone XS lookup
         out["xs_lookup"] = xs_lookup_bench(mc, decks, xs)
 
