@@ -1255,6 +1255,11 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                     if (st != ST_COLL) break;
                     st = collision_one<kTables, kGrad, false, true>(A, s, &R);
                     if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
+                    // once fewer than a quarter of the wave's lanes are still
+                    // flying, the rest stop too and fly on next iteration (the
+                    // wave would otherwise idle most lanes; every particle's
+                    // events keep their order either way)
+                    if (__popcll(__ballot(1)) < 16) break;
                     if constexpr (kTables) {
                         // Sigma_t and its grid interval at the energy the collision left
                         int g = 0;
