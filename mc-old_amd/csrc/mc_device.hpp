@@ -31,6 +31,13 @@ enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL =
 
 constexpr int kTouched = 8;        // per-slot list of tally bins touched by the history
 
+// Split copy i (from 0) of a particle starts (i + 1) 2^kCopyJumpLog2 draws
+// ahead of the particle's stream.  A history's stream is the block of
+// kRnStride draws at key * kRnStride; 2^50 draws are 7.4e9 such blocks, so a
+// copy's stream lies beyond the blocks of every key below 7.4e9, and nested
+// copies add further multiples (2048 before the 2^61-draw period wraps).
+constexpr int kCopyJumpLog2 = 50;
+
 struct DNuc {            // device nuclide (deduplicated across problems)
     double A;
     double xs_s, xs_c, xs_t;
@@ -174,7 +181,7 @@ struct KArgs {
     const double* src_cdf;        // per problem: src_count + 1 entries at cdf_off
     const int32_t* cdf_off;
     const uint64_t* jump;         // LCG jump multipliers, 256 per byte of a history key
-    uint64_t jump40;              // g^(2^40): a split copy's stream starts (i + 1) 2^40 draws on
+    uint64_t jump_copy;           // g^(2^kCopyJumpLog2): split copy i starts (i + 1) 2^50 draws on
     int32_t split_any;            // some problem can split (importance rises between cells)
     int32_t pad_split;
     const uint64_t* run_hb;       // per problem first key of this run (n_prob + 1)

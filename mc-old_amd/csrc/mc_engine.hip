@@ -665,7 +665,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.src_cdf = d_src_cdf.as<double>();
     A.cdf_off = d_cdf_off.as<int32_t>();
     A.jump = d_jump.as<uint64_t>();
-    A.jump40 = lcg_pow(kRnMult, 1ull << 40);
+    A.jump_copy = lcg_pow(kRnMult, 1ull << kCopyJumpLog2);
     A.split_any = split_any ? 1 : 0;
     if (split_any) {
         A.fam = b_fam.as<int32_t>();

@@ -929,7 +929,7 @@ __device__ __forceinline__ void weight_roulette(double& w, bool& alive, uint64_t
 // kernel's registers stay for the common path.
 // Importance splitting into n (population_control.cpp:21-49): the particle
 // goes on as one member of its history's split family and n - 1 copies of it
-// wait in the copy ring for a slot; copy i (from 0) starts (i + 1) 2^40 draws
+// wait in the copy ring for a slot; copy i (from 0) starts (i + 1) 2^50 draws
 // ahead of the particle's stream.  The family's members score into the tally
 // row of the slot where the family began, and the last member to end closes
 // it out.  Gradient runs do not split (the count reports it).
@@ -973,7 +973,7 @@ __device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, 
     }
     uint64_t r = rng;
     for (uint32_t i = 0; i < nc; ++i) {
-        r = (r * A.jump40) & kRnMask;
+        r = (r * A.jump_copy) & kRnMask;
         const uint64_t k = (t0 + i) % A.ccap;
         A.cx[k] = p.x; A.cy[k] = p.y; A.cz[k] = p.z;
         A.cu[k] = u.x; A.cv[k] = u.y; A.cw[k] = u.z;

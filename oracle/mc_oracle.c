@@ -6,13 +6,17 @@
  * reference, all shared with the GPU engine:
  *   - the implicit-fission block of collision() (general.cpp:135-151) is
  *     absent: non-multiplying media only; it consumed two random numbers;
- *   - scattering is elastic with the target at rest (the zero-temperature
- *     limit of ReactionScatter::sample, Reaction.cpp:27-118); a nuclide with
- *     A >= MC_MASS_INFINITE scatters isotropically in the lab (one-speed);
+ *   - elastic scattering is the reference's free-gas model at 293.6 K
+ *     (ReactionScatter::sample, Reaction.cpp:27-118) or, where a deck's
+ *     nuclide asks for kinematics="target_at_rest", its zero-temperature
+ *     limit; a nuclide with A >= MC_MASS_INFINITE scatters isotropically in
+ *     the lab without changing energy (one-speed);
  *   - log() and cos/sin(2 pi xi) are the project's own routines below, so
  *     that CPU and GPU histories are bit-identical;
  *   - every history owns an LCG stream started by RN_init_particle
- *     (Random.cpp:196-204) at key = history index, instead of one global stream.
+ *     (Random.cpp:196-204) at key = history index, instead of one global
+ *     stream; a split copy's stream starts 2^50 draws per copy ahead of its
+ *     parent's (see cell_importance).
  *
  * Build: gcc -O2 -ffp-contract=off (no FMA contraction: bit-exactness).
  */
@@ -577,7 +581,12 @@ static void bank_push(ostate* st, const opart* P)
 }
 
 /* population_control.cpp:21-49.  Split copies take fresh streams: copy i
- * starts i * 2^40 draws ahead of the parent's current state.            */
+ * (from 0) starts (i + 1) 2^50 draws ahead of the parent's current state.
+ * A history's stream is the block of RN_STRIDE draws at key * RN_STRIDE;
+ * 2^50 draws are 7.4e9 blocks, so a copy's stream lies beyond the blocks of
+ * every key below 7.4e9 (runs whose keys stay there cannot meet a copy's
+ * stream), and nested copies add further multiples of 2^50 (2048 of them
+ * before the 2^61-draw period wraps).                                     */
 static void cell_importance(ostate* st, opart* P)
 {
     const mc_problem_desc* d = st->d;
@@ -593,7 +602,7 @@ static void cell_importance(ostate* st, opart* P)
         P->w = P->w / (double)n;
         for (int i = 0; i < n - 1; ++i) {
             opart C = *P;
-            C.rng = mco_skip_ahead(P->rng, (int64_t)(i + 1) << 40);
+            C.rng = mco_skip_ahead(P->rng, (int64_t)(i + 1) << MCO_COPY_JUMP_LOG2);
             bank_push(st, &C);
         }
     }
@@ -710,8 +719,30 @@ void mco_free_gas_scatter(double A, double* E, double u[3], uint64_t* rng)
     for (int k = 0; k < 3; ++k) u[k] = vl[k] / sp;
 }
 
-/* elastic scatter: free gas (above) or target at rest, its T -> 0 limit:
- * mu_cm isotropic (Distribution.cpp:74-77), then scatter_direction          */
+/* Elastic scatter off a target at rest, the T -> 0 limit of
+ * ReactionScatter::sample (Reaction.cpp:27-118): the centre-of-mass cosine
+ * mu_cm = 2 xi - 1 (DistributionIsotropicScatter, Distribution.cpp:74-77),
+ * then two-body kinematics with target mass ratio A,
+ *   E' = E (A^2 + 2 A mu_cm + 1) / (A + 1)^2,
+ *   mu_lab = (1 + A mu_cm) / sqrt(A^2 + 2 A mu_cm + 1),
+ * and the new direction from scatter_direction (Algorithm.cpp:67-101).  An
+ * infinitely heavy target (A >= MC_MASS_INFINITE) keeps E and scatters with
+ * mu_lab = mu_cm (one-speed media).                                         */
+void mco_elastic_target_at_rest(double A, double* E, double u[3], uint64_t* rng)
+{
+    const double mu_cm = 2.0 * mco_urand(rng) - 1.0;
+    double mu = mu_cm;
+    if (A < MC_MASS_INFINITE) {
+        const double q = A * A + 2.0 * A * mu_cm + 1.0;
+        *E = *E * q / ((A + 1.0) * (A + 1.0));
+        mu = (1.0 + A * mu_cm) / sqrt(q);
+    }
+    double o[3];
+    mco_scatter_direction(u, mu, rng, o);
+    u[0] = o[0]; u[1] = o[1]; u[2] = o[2];
+}
+
+/* elastic scatter of a collision: free gas (above) or target at rest */
 static void scatter(ostate* st, opart* P, int nuc)
 {
     const mc_nuclide_desc* N = &st->d->nuclides[nuc];
@@ -719,17 +750,7 @@ static void scatter(ostate* st, opart* P, int nuc)
         mco_free_gas_scatter(N->A, &P->E, P->u, &P->rng);
         return;
     }
-    const double mu_cm = 2.0 * mco_urand(&P->rng) - 1.0;
-    double mu = mu_cm;
-    if (N->A < MC_MASS_INFINITE) {
-        const double A = N->A;
-        const double q = A * A + 2.0 * A * mu_cm + 1.0;
-        P->E = P->E * q / ((A + 1.0) * (A + 1.0));
-        mu = (1.0 + A * mu_cm) / sqrt(q);
-    }
-    double o[3];
-    mco_scatter_direction(P->u, mu, &P->rng, o);
-    P->u[0] = o[0]; P->u[1] = o[1]; P->u[2] = o[2];
+    mco_elastic_target_at_rest(N->A, &P->E, P->u, &P->rng);
 }
 
 /* general.cpp:121-163 without the fission block */
@@ -816,6 +837,19 @@ static void random_walk(ostate* st, opart* P)
     if (st->gout) close_particle(st, P);
 }
 
+/* DistributionIsotropicDirection::sample (Distribution.cpp:78-92): polar
+ * cosine mu = 2 xi1 - 1 on the x axis, azimuth 2 pi xi2 in the y-z plane.  */
+void mco_isotropic_direction(uint64_t* rng, double u[3])
+{
+    const double mu = 2.0 * mco_urand(rng) - 1.0;
+    double ca, sa;
+    mco_sincos_2pi(mco_urand(rng), &ca, &sa);
+    const double c = sqrt(1.0 - mu * mu);
+    u[1] = ca * c;
+    u[2] = sa * c;
+    u[0] = mu;
+}
+
 /* handler.cpp:11-48 for one cycle of fixed source */
 int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
             uint64_t n_hist, double* sum, double* sq, uint64_t* flights,
@@ -852,14 +886,8 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
         P.p[0] = src->pos[0]; P.p[1] = src->pos[1]; P.p[2] = src->pos[2];
         if (src->dir_kind == MC_DIR_DELTA) {
             P.u[0] = src->dir[0]; P.u[1] = src->dir[1]; P.u[2] = src->dir[2];
-        } else {   /* Distribution.cpp:78-92 */
-            const double mu = 2.0 * mco_urand(&P.rng) - 1.0;
-            double ca, sa;
-            mco_sincos_2pi(mco_urand(&P.rng), &ca, &sa);
-            const double c = sqrt(1.0 - mu * mu);
-            P.u[1] = ca * c;
-            P.u[2] = sa * c;
-            P.u[0] = mu;
+        } else {
+            mco_isotropic_direction(&P.rng, P.u);
         }
         if (src->energy_kind == MC_ENERGY_UNIFORM)   /* Distribution.cpp:30-33 */
             P.E = src->energy[0] + mco_urand(&P.rng) * (src->energy[1] - src->energy[0]);

@@ -38,6 +38,17 @@ void   mco_sincos_2pi(double xi, double* c, double* s);
 /* --- Reaction.cpp:27-118 ------------------------------------------------------ */
 /* free-gas elastic scatter at 293.6 K: updates *E (eV) and the direction u */
 void   mco_free_gas_scatter(double A, double* E, double u[3], uint64_t* rng);
+/* elastic scatter off a target at rest (the T -> 0 limit): updates *E and u */
+void   mco_elastic_target_at_rest(double A, double* E, double u[3], uint64_t* rng);
+
+/* --- Distribution.cpp:78-92 ------------------------------------------------------ */
+/* isotropic direction: u[0] = 2 xi1 - 1, azimuth 2 pi xi2 around x           */
+void   mco_isotropic_direction(uint64_t* rng, double u[3]);
+
+/* Split copy i (from 0) of a particle starts (i + 1) << MCO_COPY_JUMP_LOG2
+ * draws ahead of the particle's stream (population_control.cpp:21-49 pushes
+ * copies; the per-history streams need a place for theirs).                  */
+#define MCO_COPY_JUMP_LOG2 50
 
 /* --- Geometry.cpp ------------------------------------------------------------ */
 double mco_surface_eval(const mc_surface_desc* S, const double p[3]);
