@@ -47,7 +47,9 @@ struct DNuc {            // device nuclide (deduplicated across problems)
 
 struct DGrid {
     int32_t e_off, n;             // energies in grid_E[e_off .. e_off+n)
-    int32_t b_off, nb;            // buckets in bucket[b_off .. b_off+nb]
+    int32_t b_off, nb;            // buckets in bucket[b_off .. b_off+nb]; b_off < 0:
+                                  // no bucket table (a grid whose energies decrease
+                                  // somewhere: the search runs over the whole grid)
     int64_t key_base;             // (bits(E_0) >> kBucketShift)
 };
 
@@ -259,9 +261,13 @@ struct KArgs {
 // ------------------------------------------------------------------ XS lookup
 
 // Nuclide::checkE's binary_search (Algorithm.cpp:46-64) narrowed by the
-// grid's bucket table: the result equals the full search exactly.
+// grid's bucket table: the result equals the full search exactly.  A grid
+// that is not non-decreasing (the reference reads such files as they are)
+// has no table and takes the full search, whose probes alone define the
+// result there.
 __device__ __forceinline__ int grid_index(const KArgs& A, const DGrid& g, double E)
 {
+    if (g.b_off < 0) return binary_search(E, A.grid_E + g.e_off, g.n);
     const int64_t k = (int64_t)(double_to_bits(E) >> kBucketShift) - g.key_base;
     if (k < 0) return -1;
     if (k >= g.nb) return g.n - 1;

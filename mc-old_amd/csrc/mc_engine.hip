@@ -87,7 +87,9 @@ struct Engine::Impl {
     hipStream_t stream = nullptr;
     int n_cu = 0;
     mc_engine_cfg cfg{};
-    std::vector<const Problem*> probs;
+    // per problem, copied at creation (the engine does not keep the problems)
+    std::vector<uint64_t> deck_hist;
+    int n_mat0 = 0;                    // materials of problem 0 (standalone lookups)
 
     // concatenated host data
     std::vector<mc_surface_desc> surf;
@@ -150,6 +152,7 @@ struct Engine::Impl {
     int graph_iters[2] = {0, 0};
     uint64_t generation = 0, graph_gen[2] = {~0ull, ~0ull};
     const void* graph_trace[2] = {nullptr, nullptr};   // trace pointer captured in each graph
+    uint64_t graph_tcount[2] = {0, 0};                 // and the trace count (record stride)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;     // per-kernel timing events (profile mode)
     mc_kernel_stats kstats{};
@@ -185,7 +188,8 @@ struct Engine::Impl {
 // ---------------------------------------------------------------- building
 void Engine::Impl::build(const std::vector<const Problem*>& ps)
 {
-    probs = ps;
+    for (const Problem* P : ps) deck_hist.push_back(P->n_histories);
+    n_mat0 = (int)ps[0]->materials.size();
     std::map<std::vector<double>, int> grid_ids;
     std::map<std::string, int> nuc_ids;
     for (size_t p = 0; p < ps.size(); ++p) {
@@ -228,20 +232,30 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
                     G.e_off = (int)grid_E.size();
                     G.n = (int)E.size();
                     grid_E.insert(grid_E.end(), E.begin(), E.end());
-                    // bucket table over the bit pattern: lo[k] = binary_search(low(k))
-                    const int64_t k0 = (int64_t)(double_to_bits(E.front()) >> kBucketShift);
-                    const int64_t k1 = (int64_t)(double_to_bits(E.back()) >> kBucketShift);
-                    G.key_base = k0;
-                    G.nb = (int)(k1 - k0 + 1);
-                    G.b_off = (int)bucket.size();
-                    for (int64_t k = 0; k <= G.nb; ++k) {
-                        const double low = bits_to_double((uint64_t)(k0 + k) << kBucketShift);
-                        bucket.push_back(binary_search(low, E.data(), (int)E.size()));
+                    if (std::is_sorted(E.begin(), E.end()) && E.front() > 0.0) {
+                        // bucket table over the bit pattern: lo[k] = binary_search(low(k));
+                        // exact for non-decreasing grids (repeated energies included:
+                        // "largest i with E_i < x" is then one boundary)
+                        const int64_t k0 = (int64_t)(double_to_bits(E.front()) >> kBucketShift);
+                        const int64_t k1 = (int64_t)(double_to_bits(E.back()) >> kBucketShift);
+                        G.key_base = k0;
+                        G.nb = (int)(k1 - k0 + 1);
+                        G.b_off = (int)bucket.size();
+                        for (int64_t k = 0; k <= G.nb; ++k) {
+                            const double low = bits_to_double((uint64_t)(k0 + k) << kBucketShift);
+                            bucket.push_back(binary_search(low, E.data(), (int)E.size()));
+                        }
+                    } else {
+                        G.key_base = 0;
+                        G.nb = 0;
+                        G.b_off = -1;                 // full search (grid_index)
                     }
                     grid.push_back(G);
                     g = grid_ids.emplace(E, (int)grid.size() - 1).first;
-                    if (emin == 0 || E.front() < emin) emin = E.front();
-                    emax = std::max(emax, E.back());
+                    const double lo = *std::min_element(E.begin(), E.end());
+                    const double hi = *std::max_element(E.begin(), E.end());
+                    if (lo > 0.0 && (emin == 0 || lo < emin)) emin = lo;
+                    emax = std::max(emax, hi);
                 }
                 D.grid = g->second;
                 D.tab = (int)tab_s.size();
@@ -419,7 +433,7 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         L.rec_off = -1;
         L.te_off = -1;
         L.grid = mat_grid[m];
-        if (R.t_off >= 0 && n <= kFastNuc) {
+        if (R.t_off >= 0 && n <= kFastNuc && grid[mat_grid[m]].b_off >= 0) {
             const DGrid& G = grid[mat_grid[m]];
             int crowd = 0;
             for (int k = 0; k < G.nb; ++k) crowd = std::max(crowd, bucket[G.b_off + k + 1] - bucket[G.b_off + k]);
@@ -797,7 +811,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
     const bool use_graph = cfg.use_graph && !profile;
     if (use_graph) {
         if (!graph[gi] || graph_gen[gi] != generation || graph_iters[gi] != K ||
-            graph_trace[gi] != (const void*)A.trace) {
+            graph_trace[gi] != (const void*)A.trace || graph_tcount[gi] != A.trace_count) {
             if (graph[gi]) { HIP_CHECK(hipGraphExecDestroy(graph[gi])); graph[gi] = nullptr; }
             hipGraph_t gr;
             HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -808,6 +822,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             graph_gen[gi] = generation;
             graph_iters[gi] = K;
             graph_trace[gi] = (const void*)A.trace;
+            graph_tcount[gi] = A.trace_count;
         }
     }
     if (profile && kev.empty()) {
@@ -917,7 +932,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     const int np = (int)prob.size();
     std::vector<uint64_t> nh(np), hb(np + 1, 0);
     for (int p = 0; p < np; ++p) {
-        nh[p] = (run && run->n_histories) ? run->n_histories[p] : probs[p]->n_histories;
+        nh[p] = (run && run->n_histories) ? run->n_histories[p] : deck_hist[p];
         if (nh[p] < 2) throw Error(MC_ERR_ARGUMENT, "each problem needs at least 2 histories");
         hb[p + 1] = hb[p] + nh[p];
     }
@@ -1212,7 +1227,7 @@ void Engine::xs_lookup(const double* E, const int32_t* mat, double* out, uint64_
 {
     HIP_CHECK(hipSetDevice(d->device));
     for (uint64_t i = 0; i < n; ++i)
-        if (mat[i] < -1 || mat[i] >= (int32_t)d->probs[0]->materials.size())
+        if (mat[i] < -1 || mat[i] >= (int32_t)d->n_mat0)
             throw Error(MC_ERR_ARGUMENT, "material index out of range");
     DevBuf bE, bM, bO;
     bE.alloc(8 * n);
@@ -1235,7 +1250,7 @@ void Engine::xs_bench(uint64_t n, int iters, uint64_t seed, double* sec, double*
     HIP_CHECK(hipSetDevice(d->device));
     if (!d->has_tables) throw Error(MC_ERR_ARGUMENT, "no tabulated nuclides to look up");
     std::vector<int32_t> mats;
-    for (int m = 0; m < (int)d->probs[0]->materials.size(); ++m) mats.push_back(m);
+    for (int m = 0; m < d->n_mat0; ++m) mats.push_back(m);
     DevBuf bE, bM, bO, bmats;
     bE.alloc(8 * n);
     bM.alloc(4 * n);
@@ -1295,7 +1310,7 @@ void Engine::set_cfg(const mc_engine_cfg& c)
 mc_engine_cfg Engine::cfg() const { return d->cfg; }
 
 int Engine::n_problems() const { return (int)d->prob.size(); }
-uint64_t Engine::deck_histories(int p) const { return d->probs[p]->n_histories; }
+uint64_t Engine::deck_histories(int p) const { return d->deck_hist[p]; }
 
 void selftest_math(const double* in, double* out, uint64_t n)
 {

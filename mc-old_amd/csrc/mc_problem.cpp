@@ -75,6 +75,13 @@ void Problem::finalize()
     desc.seed = seed;
 }
 
+static void need_finite(const std::vector<double>& E, const std::string& path)
+{
+    if (E.empty()) throw Error(MC_ERR_INPUT, "no cross-section rows in " + path);
+    for (double e : E)
+        if (!std::isfinite(e)) throw Error(MC_ERR_INPUT, "non-finite energy in " + path);
+}
+
 void load_xs_file(const std::string& path, double& A, std::vector<double>& E,
                   std::vector<double>& s, std::vector<double>& c, std::vector<double>& f)
 {
@@ -110,8 +117,14 @@ void load_xs_file(const std::string& path, double& A, std::vector<double>& E,
         c.push_back(r[2]);
         f.push_back(r[3]);
     }
-    for (size_t k = 1; k < E.size(); ++k)
-        if (!(E[k] > E[k - 1])) throw Error(MC_ERR_INPUT, "energy grid of " + path + " is not increasing");
+    // Energies are taken as they stand, as the reference's reader does
+    // (setup.cpp:346-376): its tables repeat an energy where a cross section
+    // jumps (xs_library/008016.txt, 006000.txt, 002003.txt), and
+    // binary_search's "largest i with E_i < x" (Algorithm.cpp:46-64) never
+    // picks such a zero-width interval; a file whose energies decrease
+    // somewhere (xs_library/005011.txt, by rounding) is searched exactly as
+    // the reference searches it.
+    need_finite(E, path);
 }
 
 namespace {
@@ -489,6 +502,12 @@ void load_problem_xml(Problem& P, const std::string& xml, const std::string& xs_
             D.reactions = r == "scatter" ? MC_REACTION_SCATTER : r == "capture" ? MC_REACTION_CAPTURE
                         : r == "total" ? (MC_REACTION_SCATTER | MC_REACTION_CAPTURE) : 0;
             need(D.reactions != 0, MC_ERR_INPUT, "parameter reaction must be scatter, capture or total");
+            // one parameter per (cell, nuclide, reaction): a second one would
+            // replace the first's scaled density while both claimed its gradient
+            for (const mc_param_desc& O : P.params)
+                need(!(O.material == m && O.nuc_slot == slot && (O.reactions & D.reactions)), MC_ERR_INPUT,
+                     "two parameters scale the " + r + " cross section of nuclide '" + q->value("nuclide") +
+                         "' in cell " + P.cell_names[c]);
             P.params.push_back(D);
         }
     }

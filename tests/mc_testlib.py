@@ -230,3 +230,178 @@ def free_gas_slab(n: int, L: float = 5.0, E0: float = 1.0, kinematics: str = "fr
     </estimator>
 </estimators>
 """
+
+
+REFERENCE_DECK = os.path.join(ROOT, "tests", "golden", "shielding_vReduction")
+
+
+def reference_shielding_deck():
+    """The reference's own examples/shielding_vReduction deck with its
+    xs_library data (O-16, H-1, He-3, C, B-10, B-11: repeated energies, one
+    file whose energies decrease twice, 5- and 6-column rows), kept as a
+    fixture: water, a B4C shield and a He-3 cylinder_z detector, 2 MeV point
+    source, free-gas scattering (the reference's default)."""
+    return mc.Problem.from_xml(REFERENCE_DECK, os.path.join(REFERENCE_DECK, "xs_library"))
+
+
+def bc_box_deck(n: int) -> str:
+    """Boundary conditions: a box whose x = 0 face is reflective (plane_x),
+    whose top is a reflective generic plane 0.2x + 0.1y + z = 3, whose x = 6
+    face is vacuum (kills before scoring: its tally stays 0) and whose y and
+    z = -3 faces lead to importance-0 cells.  Target-at-rest scattering off
+    A = 12, isotropic 1 keV point source; surface, track-length and collision
+    estimators."""
+    return f"""<?xml version='1.0' encoding='UTF-8'?>
+<simulation>
+    <description name="boundary conditions" samples="{n}" seed="3"/>
+</simulation>
+<nuclides>
+    <nuclide name="m" A="12.0" kinematics="target_at_rest">
+        <scatter xs="2.0"/>
+        <capture xs="0.1"/>
+    </nuclide>
+</nuclides>
+<materials>
+    <material name="mat">
+        <nuclide name="m" density="0.3"/>
+    </material>
+</materials>
+<surfaces>
+    <plane_x name="px0" x="0.0" bc="reflective"/>
+    <plane_x name="px1" x="6.0" bc="vacuum"/>
+    <plane_y name="py0" y="-3.0"/>
+    <plane_y name="py1" y="3.0"/>
+    <plane_z name="pz0" z="-3.0"/>
+    <plane name="top" a="0.2" b="0.1" c="1.0" d="3.0" bc="reflective"/>
+</surfaces>
+<cells>
+    <cell name="in" material="mat">
+        <surface name="px0" sense="+1"/>
+        <surface name="px1" sense="-1"/>
+        <surface name="py0" sense="+1"/>
+        <surface name="py1" sense="-1"/>
+        <surface name="pz0" sense="+1"/>
+        <surface name="top" sense="-1"/>
+    </cell>
+    <cell name="ylo" importance="0.0">
+        <surface name="py0" sense="-1"/>
+    </cell>
+    <cell name="yhi" importance="0.0">
+        <surface name="py1" sense="+1"/>
+    </cell>
+    <cell name="zlo" importance="0.0">
+        <surface name="pz0" sense="-1"/>
+    </cell>
+</cells>
+<distributions>
+    <delta name="e" datatype="double" val="1000.0"/>
+</distributions>
+<sources>
+    <point x="1.0" y="0.0" z="0.0" energy="e"/>
+</sources>
+<estimators>
+    <estimator name="vacuum face" scores="cross">
+        <surface name="px1"/>
+    </estimator>
+    <estimator name="reflective faces" scores="cross">
+        <surface name="px0"/>
+        <surface name="top"/>
+    </estimator>
+    <estimator name="leak" scores="cross">
+        <surface name="py0"/>
+        <surface name="py1"/>
+        <surface name="pz0"/>
+    </estimator>
+    <estimator name="tl" type="TL" scores="flux absorption total">
+        <cell name="in"/>
+    </estimator>
+    <estimator name="col" type="C" scores="flux scatter">
+        <cell name="in"/>
+    </estimator>
+</estimators>
+"""
+
+
+def quadric_deck(n: int) -> str:
+    """Quadrics: a cylinder_x (r = 4, 0 <= x <= 20) crossed by an absorbing
+    cylinder_z rod (r = 1.5 at x = 10), with a void sphere (r = 1 at x = 4);
+    free-gas hydrogen-like scattering, isotropic 100 eV source."""
+    return f"""<?xml version='1.0' encoding='UTF-8'?>
+<simulation>
+    <description name="quadrics" samples="{n}" seed="5"/>
+</simulation>
+<nuclides>
+    <nuclide name="h" A="1.0">
+        <scatter xs="10.0"/>
+        <capture xs="0.2"/>
+    </nuclide>
+    <nuclide name="b" A="10.0" kinematics="target_at_rest">
+        <scatter xs="2.0"/>
+        <capture xs="30.0"/>
+    </nuclide>
+</nuclides>
+<materials>
+    <material name="mod">
+        <nuclide name="h" density="0.05"/>
+    </material>
+    <material name="abs">
+        <nuclide name="b" density="0.02"/>
+        <nuclide name="h" density="0.01"/>
+    </material>
+</materials>
+<surfaces>
+    <cylinder_x name="cx" y="0.0" z="0.0" r="4.0"/>
+    <plane_x name="p0" x="0.0"/>
+    <plane_x name="p1" x="20.0"/>
+    <cylinder_z name="cz" x="10.0" y="0.0" r="1.5"/>
+    <sphere name="sp" x="4.0" y="0.0" z="0.0" r="1.0"/>
+</surfaces>
+<cells>
+    <cell name="rod" material="abs">
+        <surface name="cz" sense="-1"/>
+        <surface name="cx" sense="-1"/>
+    </cell>
+    <cell name="ball">
+        <surface name="sp" sense="-1"/>
+    </cell>
+    <cell name="body" material="mod">
+        <surface name="cx" sense="-1"/>
+        <surface name="p0" sense="+1"/>
+        <surface name="p1" sense="-1"/>
+        <surface name="cz" sense="+1"/>
+        <surface name="sp" sense="+1"/>
+    </cell>
+    <cell name="side" importance="0.0">
+        <surface name="cx" sense="+1"/>
+    </cell>
+    <cell name="back" importance="0.0">
+        <surface name="p0" sense="-1"/>
+    </cell>
+    <cell name="front" importance="0.0">
+        <surface name="p1" sense="+1"/>
+    </cell>
+</cells>
+<distributions>
+    <delta name="e" datatype="double" val="100.0"/>
+</distributions>
+<sources>
+    <point x="2.0" y="0.0" z="0.0" energy="e"/>
+</sources>
+<estimators>
+    <estimator name="transmission" scores="cross">
+        <surface name="p1"/>
+    </estimator>
+    <estimator name="side leak" scores="cross">
+        <surface name="cx"/>
+        <filter type="energy" grid_lethargy="1e-3 1e3 12"/>
+    </estimator>
+    <estimator name="flux" scores="flux">
+        <cell name="rod"/>
+        <cell name="ball"/>
+        <cell name="body"/>
+    </estimator>
+    <estimator name="rod absorption" type="C" scores="absorption">
+        <cell name="rod"/>
+    </estimator>
+</estimators>
+"""

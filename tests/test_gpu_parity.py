@@ -144,3 +144,17 @@ def test_error_paths_on_device():
     with pytest.raises(mc.MCError) as e:
         mc.Engine(p, bank_capacity=64).forward()
     assert e.value.status == mc.MC_ERR_CAPACITY
+
+
+def test_graph_replay_after_trace_count_changes():
+    """A captured graph holds the trace record stride: a second forward with
+    a smaller trace count on the same engine (two problems, graphs on) must
+    recapture, and each problem's records equal the oracle's."""
+    n = 3000
+    probs = [mc.Problem.from_string(decks.slab_deck(2.0, c, 8, n)) for c in (0.3, 0.8)]
+    eng = mc.Engine(probs, bank_capacity=1 << 11, use_graph=True)
+    for tc in (100, 50, 120):
+        g = eng.forward(trace=tc)
+        for p in range(2):
+            o = oracle_run(probs[p], tc, key_offset=p * n, trace=tc)
+            assert_traces_equal(g.trace[p * tc:(p + 1) * tc], o.trace, f"trace_count={tc} problem {p}")

@@ -133,3 +133,19 @@ def test_cli_usage_and_missing_device(tmp_path):
     import torch
     if not torch.cuda.is_available():
         assert r.returncode == 1 and "[ERROR 5]" in r.stderr
+
+
+def test_overlapping_parameters_rejected():
+    """One parameter per (cell, nuclide, reaction): a second scatter, or a
+    total beside a scatter, would replace the first's scaled density while
+    both claimed its gradient."""
+    base = decks.slab_deck(2.0, 0.5, 2, 100, params=("scatter", "capture"))
+    mc.Problem.from_string(base)
+    for extra in ("scatter", "total", "capture"):
+        bad = base.replace("</parameters>", f'    <parameter cell="bin0" nuclide="onespeed" reaction="{extra}"/>\n'
+                                            "</parameters>")
+        expect_status(bad, mc.MC_ERR_INPUT)
+    # the same reaction on another cell is a parameter of its own
+    other = base.replace("</parameters>", '    <parameter cell="bin1" nuclide="onespeed" reaction="total"/>\n'
+                                          "</parameters>")
+    assert mc.Problem.from_string(other).n_params == 3
