@@ -8,6 +8,8 @@
  * What each entry point replaces in the reference (ilhamv/MC-old):
  *   mc_problem_from_xml        Simulator::Simulator(io_dir)      src/simulator/setup.cpp:30-1069
  *   mc_problem_from_xml_string   (same parser, deck held in memory)
+ *   mc_problem_create          the same object graph built by a caller from
+ *                              flat arrays instead of a deck
  *   mc_problem_desc            the object graph the constructor builds
  *                              (Surfaces/Cells/Materials/Nuclides/Estimators,
  *                              include/simulator.h:118-122)
@@ -21,8 +23,10 @@
  *   mc_report_text             Simulator::report(io_dir)         src/simulator/report.cpp:9-50
  *
  * Ownership: the library owns mc_problem / mc_engine handles; the caller owns
- * every output buffer.  Threading: one handle per host thread; a handle is not
- * re-entrant.  All floating-point tallies are FP64.
+ * every output buffer.  An engine copies what it needs from its problems when
+ * it is created: a problem may be destroyed once the engines built from it
+ * exist.  Threading: one handle per host thread; a handle is not re-entrant;
+ * any number of engines per device.  All floating-point tallies are FP64.
  */
 #ifndef MC_API_H
 #define MC_API_H
@@ -170,6 +174,19 @@ typedef struct mc_problem mc_problem;
  * "./xs_library", setup.cpp:326).                                            */
 mc_status mc_problem_from_xml(const char* io_dir, const char* xs_dir, mc_problem** out);
 mc_status mc_problem_from_xml_string(const char* xml, const char* xs_dir, mc_problem** out);
+/* Builds a problem from caller-owned flat arrays (copied; the caller keeps
+ * its arrays).  Array lengths follow from the counts and index ranges of the
+ * records: cell_surf/cell_sense up to the largest surf_begin + surf_count,
+ * mat_nuclide/mat_density up to the largest nuc_begin + nuc_count, xs_* up to
+ * the largest grid_begin + grid_count, att_* up to the largest attachment
+ * range, egrid up to the largest egrid_begin + egrid_count, mat_param up to
+ * the largest param_begin + param_count.  source_cdf may be NULL (equal
+ * probabilities, Source.cpp:65-72); a source's cell < 0 is found by the
+ * cell search (general.cpp:26-34).  Every index is checked: MC_ERR_INPUT
+ * names the first that is out of range.  Names are "surface <i>", "cell
+ * <i>", "material <i>", "nuclide <i>" and "Estimator #<i+1>".
+ * Replaces building the object graph by Simulator::Simulator (setup.cpp:30-1069). */
+mc_status mc_problem_create(const mc_problem_desc* desc, mc_problem** out);
 mc_status mc_problem_get_desc(const mc_problem* p, const mc_problem_desc** out);
 /* Estimator name / score name lookups for reports (NULL when out of range). */
 const char* mc_problem_estimator_name(const mc_problem* p, int32_t e);
@@ -218,11 +235,24 @@ typedef struct mc_trace_record {   /* per-history state for CPU/GPU bit-equality
 
 typedef struct mc_run_cfg {
     int32_t mode;                  /* MC_RUN_BATCHED / MC_RUN_PLAIN             */
-    int32_t pad;
+    int32_t device_ptrs;           /* 1: the mc_tally_out arrays, trace, and
+                                      mc_backward's grad_out / grad / grad_sigma
+                                      are device pointers on the engine's device */
     const uint64_t* n_histories;   /* per problem, NULL: each deck's samples    */
     const double* theta;           /* n_params of the whole engine, NULL: 1.0   */
-    mc_trace_record* trace;        /* host buffer, NULL: off                    */
-    uint64_t trace_count;          /* histories traced per problem (first ids)  */
+    mc_trace_record* trace;        /* NULL: off                                 */
+    uint64_t trace_count;          /* histories traced per problem (the first
+                                      trace_count of the run's histories)       */
+    /* History keys: problem p runs the n_histories[p] histories of keys
+     * key_offset[p], key_offset[p] + 1, ...  A history's random-number stream
+     * is fixed by its key alone (RN_init_particle, Random.cpp:196-204), so runs
+     * over disjoint key ranges are shards of one run: their traces are the
+     * full run's and their moment sums add up to its sums.  NULL: problem p
+     * starts at the sum of the histories of the problems before it.          */
+    const uint64_t* key_offset;
+    void* stream;                  /* hipStream_t the run is issued on (NULL: the
+                                      engine's own stream); the call returns once
+                                      the run's work on it has finished        */
 } mc_run_cfg;
 
 typedef struct mc_tally_out {      /* caller-owned, sized engine n_bins         */

@@ -48,8 +48,9 @@ TRACE_DTYPE = np.dtype([("rng", "<u8"), ("flights", "<u8"), ("x", "<f8"), ("y", 
 
 
 class RunCfg(C.Structure):
-    _fields_ = [("mode", C.c_int32), ("pad", C.c_int32), ("n_histories", C.POINTER(C.c_uint64)),
-                ("theta", C.POINTER(C.c_double)), ("trace", C.c_void_p), ("trace_count", C.c_uint64)]
+    _fields_ = [("mode", C.c_int32), ("device_ptrs", C.c_int32), ("n_histories", C.POINTER(C.c_uint64)),
+                ("theta", C.POINTER(C.c_double)), ("trace", C.c_void_p), ("trace_count", C.c_uint64),
+                ("key_offset", C.POINTER(C.c_uint64)), ("stream", C.c_void_p)]
 
 
 class TallyOut(C.Structure):
@@ -101,6 +102,7 @@ EXPORTS = {
     "mc_version": (C.c_char_p, []),
     "mc_problem_from_xml": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     "mc_problem_from_xml_string": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mc_problem_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mc_problem_get_desc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mc_problem_estimator_name": (C.c_char_p, [C.c_void_p, C.c_int32]),
     "mc_problem_destroy": (None, [C.c_void_p]),
@@ -168,6 +170,14 @@ class Problem:
     def from_xml(cls, io_dir: str, xs_dir: str | None = None) -> "Problem":
         h = C.c_void_p()
         _check(lib.mc_problem_from_xml(io_dir.encode(), xs_dir.encode() if xs_dir else None, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_desc(cls, desc_ptr: int) -> "Problem":
+        """mc_problem_create: a problem from flat arrays (an mc_problem_desc at
+        desc_ptr, e.g. another problem's desc_ptr), copied."""
+        h = C.c_void_p()
+        _check(lib.mc_problem_create(C.c_void_p(desc_ptr), C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -254,10 +264,17 @@ class Engine:
         _check(lib.mc_engine_param_offset(self._h, p, C.byref(off), C.byref(n)))
         return slice(off.value, off.value + n.value)
 
-    def _run_cfg(self, n_histories, theta, trace, mode):
+    def _run_cfg(self, n_histories, theta, trace, mode, key_offset=None, stream=None):
         run = RunCfg()
         run.mode = mode
         keep = []
+        if key_offset is not None:
+            ko = np.ascontiguousarray(np.broadcast_to(np.asarray(key_offset, dtype=np.uint64),
+                                                      (len(self.problems),)))
+            keep.append(ko)
+            run.key_offset = ko.ctypes.data_as(C.POINTER(C.c_uint64))
+        if stream is not None:
+            run.stream = C.c_void_p(int(stream))
         if n_histories is not None:
             nh = np.ascontiguousarray(np.broadcast_to(np.asarray(n_histories, dtype=np.uint64),
                                                       (len(self.problems),)))
@@ -277,8 +294,11 @@ class Engine:
             run.trace_count = trace
         return run, keep, nh, tr
 
-    def forward(self, n_histories=None, theta=None, trace: int = 0, mode: int = MC_RUN_BATCHED) -> TallyResult:
-        run, keep, nh, tr = self._run_cfg(n_histories, theta, trace, mode)
+    def forward(self, n_histories=None, theta=None, trace: int = 0, mode: int = MC_RUN_BATCHED,
+                key_offset=None, stream=None) -> TallyResult:
+        """key_offset: per problem (or one for all) first history key of the run;
+        stream: a hipStream_t handle (int) to issue the run on."""
+        run, keep, nh, tr = self._run_cfg(n_histories, theta, trace, mode, key_offset, stream)
         nb = max(1, self.n_bins)
         s, q, m, u = (np.zeros(nb) for _ in range(4))
         out = TallyOut(_dptr(s), _dptr(q), _dptr(m), _dptr(u), 0, 0.0)
@@ -286,8 +306,9 @@ class Engine:
         return TallyResult(m[:self.n_bins], u[:self.n_bins], s[:self.n_bins], q[:self.n_bins],
                            out.flights, out.seconds, nh, tr)
 
-    def backward(self, grad_out, n_histories=None, theta=None, mode: int = MC_RUN_BATCHED) -> TallyResult:
-        run, keep, nh, tr = self._run_cfg(n_histories, theta, 0, mode)
+    def backward(self, grad_out, n_histories=None, theta=None, mode: int = MC_RUN_BATCHED,
+                 key_offset=None, stream=None) -> TallyResult:
+        run, keep, nh, tr = self._run_cfg(n_histories, theta, 0, mode, key_offset, stream)
         go = np.ascontiguousarray(grad_out, dtype=np.float64)
         assert go.shape == (self.n_bins,)
         nb = max(1, self.n_bins)
@@ -298,6 +319,34 @@ class Engine:
         _check(lib.mc_backward(self._h, C.byref(run), _dptr(go), _dptr(g), _dptr(gs), C.byref(out)))
         return TallyResult(m[:self.n_bins], u[:self.n_bins], s[:self.n_bins], q[:self.n_bins],
                            out.flights, out.seconds, nh, None, g[:self.n_params], gs[:self.n_params])
+
+    def forward_device(self, sum_ptr: int, sq_ptr: int, mean_ptr: int, uncer_ptr: int, n_histories=None,
+                       theta=None, key_offset=None, stream=None, trace_ptr: int = 0, trace: int = 0) -> dict:
+        """mc_forward with device-pointer outputs (run.device_ptrs = 1): the
+        moment arrays (engine n_bins doubles each, 0 to skip one) and the trace
+        records (trace per problem) are written on the device."""
+        run, keep, nh, _ = self._run_cfg(n_histories, theta, 0, MC_RUN_BATCHED, key_offset, stream)
+        run.device_ptrs = 1
+        if trace:
+            run.trace = C.c_void_p(trace_ptr)
+            run.trace_count = trace
+        dp = C.POINTER(C.c_double)
+        out = TallyOut(C.cast(C.c_void_p(sum_ptr), dp), C.cast(C.c_void_p(sq_ptr), dp),
+                       C.cast(C.c_void_p(mean_ptr), dp), C.cast(C.c_void_p(uncer_ptr), dp), 0, 0.0)
+        _check(lib.mc_forward(self._h, C.byref(run), C.byref(out)))
+        return {"flights": out.flights, "seconds": out.seconds}
+
+    def backward_device(self, grad_out_ptr: int, grad_ptr: int, grad_sigma_ptr: int, n_histories=None,
+                        theta=None, key_offset=None, stream=None) -> dict:
+        """mc_backward with device-pointer weights and gradient outputs."""
+        run, keep, nh, _ = self._run_cfg(n_histories, theta, 0, MC_RUN_BATCHED, key_offset, stream)
+        run.device_ptrs = 1
+        out = TallyOut()
+        dp = C.POINTER(C.c_double)
+        _check(lib.mc_backward(self._h, C.byref(run), C.cast(C.c_void_p(grad_out_ptr), dp),
+                               C.cast(C.c_void_p(grad_ptr), dp), C.cast(C.c_void_p(grad_sigma_ptr), dp),
+                               C.byref(out)))
+        return {"flights": out.flights, "seconds": out.seconds}
 
     def kernel_stats(self, reset: bool = True) -> dict:
         """Per-kernel device seconds, launches and particles processed

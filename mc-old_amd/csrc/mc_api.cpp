@@ -81,6 +81,16 @@ mc_status mc_problem_from_xml_string(const char* xml, const char* xs_dir, mc_pro
     });
 }
 
+mc_status mc_problem_create(const mc_problem_desc* desc, mc_problem** out)
+{
+    return guarded([&] {
+        if (!desc || !out) throw mc::Error(MC_ERR_ARGUMENT, "null argument");
+        auto p = std::make_unique<mc_problem>();
+        mc::load_problem_desc(p->P, *desc);
+        *out = p.release();
+    });
+}
+
 mc_status mc_problem_get_desc(const mc_problem* p, const mc_problem_desc** out)
 {
     return guarded([&] {

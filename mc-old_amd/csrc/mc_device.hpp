@@ -186,7 +186,10 @@ struct KArgs {
     uint64_t jump_copy;           // g^(2^kCopyJumpLog2): split copy i starts (i + 1) 2^50 draws on
     int32_t split_any;            // some problem can split (importance rises between cells)
     int32_t pad_split;
-    const uint64_t* run_hb;       // per problem first key of this run (n_prob + 1)
+    const uint64_t* run_hb;       // per problem first run key of this run (n_prob + 1)
+    // per problem g^(stride (key_offset - run_hb)) mod 2^63: moves a history
+    // from its run key to the key the caller gave it
+    const uint64_t* run_jd;
     // per run: {shift, n, then for j = 0..n the problem holding key
     // min(j << shift, last key)}; n <= 65536 blocks of 2^shift keys
     const int32_t* hb_idx;

@@ -35,6 +35,8 @@ __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, d
                             const int32_t* mats, int32_t nmat, uint64_t seed);
 __global__ void k_fill_f64(double* p, uint64_t n, double v);
 __global__ void k_reduce_rows(const double* part, uint64_t rows, uint64_t n, double* out);
+__global__ void k_moments(const double* sum, const double* sq, const double* N, uint64_t n, double* osum,
+                          double* osq, double* omean, double* ouncer);
 __global__ void k_fill_u8(uint8_t* p, uint64_t n, uint8_t v);
 __global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v);
 __global__ void k_selftest_math(const double* in, double* out, uint64_t n);
@@ -84,7 +86,8 @@ static uint64_t lcg_pow(uint64_t g, uint64_t e)
 
 struct Engine::Impl {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the engine's own stream (graphs are captured on it)
+    hipStream_t cur = nullptr;         // the stream of the current run
     int n_cu = 0;
     mc_engine_cfg cfg{};
     // per problem, copied at creation (the engine does not keep the problems)
@@ -129,7 +132,7 @@ struct Engine::Impl {
         d_mat_ns, d_mat_nc, d_mat_tconst, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nfold, d_mfold, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
         d_tab_c, d_tab_t, d_est, d_att_est, d_att_idx, d_egrid, d_prob, d_src, d_src_cdf, d_cdf_off, d_jump,
-        d_run_hb, d_hb_idx;
+        d_run_hb, d_run_jd, d_hb_idx, d_binN, d_parN;
     std::vector<int32_t> h_hb_idx;    // host copy of the run's key-to-problem index
     // bank
     DevBuf b_x, b_y, b_z, b_u, b_v, b_w, b_E, b_wgt, b_sigt, b_rng, b_hid, b_cell, b_prob, b_surf, b_nfl,
@@ -543,6 +546,9 @@ void Engine::Impl::upload_static()
     upload(d_jump, jump, stream);
     if (split_any) d_wgheld.alloc(sizeof(uint32_t) * 8192);
     d_run_hb.alloc(sizeof(uint64_t) * (prob.size() + 1));
+    d_run_jd.alloc(sizeof(uint64_t) * prob.size());
+    d_binN.alloc(sizeof(double) * std::max(1, n_bins));
+    d_parN.alloc(sizeof(double) * std::max(1, n_params));
     d_hb_idx.alloc(sizeof(int32_t) * (kHbIdxMax + 3));
     t_sum.alloc(sizeof(double) * std::max(1, n_bins));
     t_sq.alloc(sizeof(double) * std::max(1, n_bins));
@@ -698,6 +704,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
         A.ccap = ring_cap;
     }
     A.run_hb = d_run_hb.as<uint64_t>();
+    A.run_jd = d_run_jd.as<uint64_t>();
     A.hb_idx = d_hb_idx.as<int32_t>();
     A.n_prob = (int32_t)prob.size();
     A.bin_stride = bin_stride;
@@ -834,14 +841,14 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
     const dim3 g = grid_dim(), b((unsigned)cfg.block);
     uint64_t drain_flights = 0, drain_coll = 0;
     auto refill = [&]() {
-        hipLaunchKernelGGL(k_refill_count, g, b, 0, stream, A);
-        if (grad) hipLaunchKernelGGL(k_refill_scatter<true>, g, b, 0, stream, A);
-        else hipLaunchKernelGGL(k_refill_scatter<false>, g, b, 0, stream, A);
+        hipLaunchKernelGGL(k_refill_count, g, b, 0, cur, A);
+        if (grad) hipLaunchKernelGGL(k_refill_scatter<true>, g, b, 0, cur, A);
+        else hipLaunchKernelGGL(k_refill_scatter<false>, g, b, 0, cur, A);
     };
     auto read_ctr = [&]() {
         HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
+        HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, cur));
+        HIP_CHECK(hipStreamSynchronize(cur));
     };
     for (uint64_t chunk = 0;; ++chunk) {
         // The tail of a run: every history has started and few particles are
@@ -854,19 +861,19 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             // the roll-over in the refill below adds the last iteration's counts
             const uint64_t tA0 = h_ctr->tA + h_ctr->nA, tC0 = h_ctr->tC + h_ctr->nC, x0 = h_ctr->xfl;
             refill();
-            if (profile) HIP_CHECK(hipEventRecord(kev[2 * kDrainSlot], stream));
+            if (profile) HIP_CHECK(hipEventRecord(kev[2 * kDrainSlot], cur));
             const uint32_t kMaxEvents = 1u << 14;
             if (has_tables) {
-                if (grad) hipLaunchKernelGGL((k_drain<true, true>), g, b, 0, stream, A, kMaxEvents);
-                else hipLaunchKernelGGL((k_drain<true, false>), g, b, 0, stream, A, kMaxEvents);
+                if (grad) hipLaunchKernelGGL((k_drain<true, true>), g, b, 0, cur, A, kMaxEvents);
+                else hipLaunchKernelGGL((k_drain<true, false>), g, b, 0, cur, A, kMaxEvents);
             } else {
-                if (grad) hipLaunchKernelGGL((k_drain<false, true>), g, b, 0, stream, A, kMaxEvents);
-                else hipLaunchKernelGGL((k_drain<false, false>), g, b, 0, stream, A, kMaxEvents);
+                if (grad) hipLaunchKernelGGL((k_drain<false, true>), g, b, 0, cur, A, kMaxEvents);
+                else hipLaunchKernelGGL((k_drain<false, false>), g, b, 0, cur, A, kMaxEvents);
             }
-            if (profile) HIP_CHECK(hipEventRecord(kev[2 * kDrainSlot + 1], stream));
+            if (profile) HIP_CHECK(hipEventRecord(kev[2 * kDrainSlot + 1], cur));
             // the drain counted its own events; the refill after it must not
             // count the drained particles again
-            hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, stream, A.ctr);
+            hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, cur, A.ctr);
             refill();
             read_ctr();
             drain_flights += h_ctr->tA - tA0 - (h_ctr->xfl - x0);
@@ -881,14 +888,14 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             if (h_ctr->nA == 0) break;
             // histories longer than the drain's event limit: back to iterations,
             // whose first roll-over must not count this refill's particles
-            hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, stream, A.ctr);
+            hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, cur, A.ctr);
             continue;
         }
         if (use_graph) {
-            HIP_CHECK(hipGraphLaunch(graph[gi], stream));
+            HIP_CHECK(hipGraphLaunch(graph[gi], cur));
         } else if (profile) {
             for (int k = 0; k < K; ++k) {
-                launch_iteration(A, grad, stream, kev.data());
+                launch_iteration(A, grad, cur, kev.data());
                 HIP_CHECK(hipEventSynchronize(kev[2 * kIterKernels - 1]));
                 for (int j = 0; j < kIterKernels; ++j) {
                     // no lookup kernel without tables, no collision kernel when the
@@ -903,7 +910,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                 kstats.items[1] += h_active_cap;
             }
         } else {
-            for (int k = 0; k < K; ++k) launch_iteration(A, grad, stream, nullptr);
+            for (int k = 0; k < K; ++k) launch_iteration(A, grad, cur, nullptr);
         }
         read_ctr();
         if (h_ctr->nA == 0) break;
@@ -929,6 +936,8 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
                        double* grad_mean, double* grad_sigma)
 {
     HIP_CHECK(hipSetDevice(device));
+    cur = (run && run->stream) ? (hipStream_t)run->stream : stream;
+    const bool dev = run && run->device_ptrs;
     const int np = (int)prob.size();
     std::vector<uint64_t> nh(np), hb(np + 1, 0);
     for (int p = 0; p < np; ++p) {
@@ -948,9 +957,9 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         if (D.reactions & MC_REACTION_SCATTER) ns[j] = th[k] * mat_dens[j];
         if (D.reactions & MC_REACTION_CAPTURE) nc[j] = th[k] * mat_dens[j];
     }
-    upload(d_theta, th, stream);
-    upload(d_mat_ns, ns, stream);
-    upload(d_mat_nc, nc, stream);
+    upload(d_theta, th, cur);
+    upload(d_mat_ns, ns, cur);
+    upload(d_mat_nc, nc, cur);
     // Sigma_t of materials whose nuclides all have constant cross sections,
     // summed as material_sigt sums it (same terms, same order)
     std::vector<double> tconst(std::max<size_t>(1, mat.size()), std::nan(""));
@@ -976,7 +985,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
             tconst[m] = sv + cv;
         }
     }
-    upload(d_mat_tconst, tconst, stream);
+    upload(d_mat_tconst, tconst, cur);
     // the collision fold's packed parameter table (1/theta as the kernels would
     // divide it: IEEE division on both sides)
     std::vector<int> owner(std::max(1, n_params), 0);
@@ -995,7 +1004,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
             F.N = mat_dens[mat[m].nuc_begin + D.nuc_slot];
             F.inv_theta = 1.0 / th[kg];
         }
-    upload(d_mpar, mpar, stream);
+    upload(d_mpar, mpar, cur);
     std::vector<DNucFold> nfold(std::max<size_t>(1, mat_nuc.size()), DNucFold{-1, -1, 0.0});
     std::vector<int32_t> mfold(std::max<size_t>(1, mat.size()), 0);
     for (size_t m = 0; m < mat.size(); ++m) {
@@ -1014,9 +1023,17 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         }
         mfold[m] = fast ? 1 : 0;
     }
-    upload(d_nfold, nfold, stream);
-    upload(d_mfold, mfold, stream);
-    HIP_CHECK(hipMemcpyAsync(d_run_hb.p, hb.data(), sizeof(uint64_t) * (np + 1), hipMemcpyHostToDevice, stream));
+    upload(d_nfold, nfold, cur);
+    upload(d_mfold, mfold, cur);
+    HIP_CHECK(hipMemcpyAsync(d_run_hb.p, hb.data(), sizeof(uint64_t) * (np + 1), hipMemcpyHostToDevice, cur));
+    // history keys: problem p's j-th history of the run has run key hb[p] + j
+    // and key key_offset[p] + j; its stream start times g^(stride (key_offset
+    // - hb)) moves it from the one to the other (exponents wrap mod 2^64, a
+    // multiple of the generator's period 2^61)
+    std::vector<uint64_t> jd(np, 1);
+    if (run && run->key_offset)
+        for (int p = 0; p < np; ++p) jd[p] = lcg_pow(kRnMult, kRnStride * (run->key_offset[p] - hb[p]));
+    HIP_CHECK(hipMemcpyAsync(d_run_jd.p, jd.data(), sizeof(uint64_t) * np, hipMemcpyHostToDevice, cur));
     {
         // key -> problem index: blocks of 2^sh keys, at most kHbIdxMax of them;
         // a key's problem lies between those of its block's two ends
@@ -1035,40 +1052,48 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
             while (p + 1 < (int)np && hb[p + 1] <= key) ++p;
             idx[2 + j] = p;
         }
-        HIP_CHECK(hipMemcpyAsync(d_hb_idx.p, idx.data(), sizeof(int32_t) * (nj + 3), hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d_hb_idx.p, idx.data(), sizeof(int32_t) * (nj + 3), hipMemcpyHostToDevice, cur));
     }
     if (grad) {
-        std::vector<double> g(gout, gout + n_bins);
-        HIP_CHECK(hipMemcpyAsync(t_gout.p, g.data(), sizeof(double) * n_bins, hipMemcpyHostToDevice, stream));
+        if (dev) {
+            HIP_CHECK(hipMemcpyAsync(t_gout.p, gout, sizeof(double) * n_bins, hipMemcpyDeviceToDevice, cur));
+        } else {
+            std::vector<double> g(gout, gout + n_bins);
+            HIP_CHECK(hipMemcpyAsync(t_gout.p, g.data(), sizeof(double) * n_bins, hipMemcpyHostToDevice, cur));
+        }
     }
-    // trace buffer
+    // trace buffer: the caller's device buffer, or the engine's, copied back
     mc_trace_record* dtrace = nullptr;
     const uint64_t tcount = (run && run->trace) ? run->trace_count : 0;
     if (tcount) {
         const size_t bytes = sizeof(mc_trace_record) * tcount * (size_t)np;
-        if (t_trace.bytes < bytes) { t_trace.alloc(bytes); ++generation; }
-        HIP_CHECK(hipMemsetAsync(t_trace.p, 0, bytes, stream));
-        dtrace = t_trace.as<mc_trace_record>();
+        if (dev) {
+            dtrace = run->trace;
+        } else {
+            if (t_trace.bytes < bytes) { t_trace.alloc(bytes); ++generation; }
+            dtrace = t_trace.as<mc_trace_record>();
+        }
+        hipLaunchKernelGGL(k_fill_u8, dim3(1024), dim3(256), 0, cur, (uint8_t*)dtrace, (uint64_t)bytes, (uint8_t)0);
     }
     ensure_partials();
     const KArgs A = args(grad, dtrace, tcount);
     const dim3 fg(1024), fb(256);
     const uint64_t nbp = part_rows * (uint64_t)std::max(1, n_bins), npp = part_rows * (uint64_t)std::max(1, n_params);
-    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.psum, nbp, 0.0);
-    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.psq, nbp, 0.0);
-    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.hist, bank_cap * (uint64_t)bin_stride, 0.0);
-    hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.tcount, bank_cap, (uint8_t)0);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.psum, nbp, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.psq, nbp, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.hist, bank_cap * (uint64_t)bin_stride, 0.0);
+    hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, cur, A.tcount, bank_cap, (uint8_t)0);
     if (grad) {
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsum, npp, 0.0);
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.pgsq, npp, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.pgsum, npp, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.pgsq, npp, 0.0);
         // every history's closeout clears the S and H entries it used, so the
         // bank starts each run cleared
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.S, lr_stride(bank_cap) * (uint64_t)par_stride, 0.0);
-        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.H, lr_stride(bank_cap) * (uint64_t)par_stride, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.S, lr_stride(bank_cap) * (uint64_t)par_stride, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.H, lr_stride(bank_cap) * (uint64_t)par_stride, 0.0);
         if (split_any)
-            hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, stream, A.famG, bank_cap * (uint64_t)par_stride, 0.0);
+            hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.famG, bank_cap * (uint64_t)par_stride, 0.0);
     }
-    HIP_CHECK(hipEventRecord(ev0, stream));
+    HIP_CHECK(hipEventRecord(ev0, cur));
     const bool plain = run && run->mode == MC_RUN_PLAIN;
     uint64_t flights = 0;
     uint32_t err_lost = 0, err_split = 0, err_ring = 0;
@@ -1079,12 +1104,12 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         // slots worth scanning: no more than the histories of this run
         c.active_cap = (uint32_t)std::min<uint64_t>(bank_cap, c.end_hist - c.next_hist);
         h_active_cap = c.active_cap;
-        HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, A.status, bank_cap, (uint8_t)ST_EMPTY);
-        hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, stream, (uint8_t*)A.wg_xfl, sizeof(uint64_t) * 8192, (uint8_t)0);
+        HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, cur));
+        hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, cur, A.status, bank_cap, (uint8_t)ST_EMPTY);
+        hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, cur, (uint8_t*)A.wg_xfl, sizeof(uint64_t) * 8192, (uint8_t)0);
         if (split_any) {
-            hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, stream, A.fam, bank_cap, -1);
-            hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, stream, A.fcount, bank_cap, 0);
+            hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.fam, bank_cap, -1);
+            hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.fcount, bank_cap, 0);
         }
         run_loop(grad, A);
         flights += h_ctr->tA;      // every entry of queue A is one flight
@@ -1092,7 +1117,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         err_split += h_ctr->err_split;
         err_ring += h_ctr->err_ring;
     }
-    HIP_CHECK(hipEventRecord(ev1, stream));
+    HIP_CHECK(hipEventRecord(ev1, cur));
     HIP_CHECK(hipEventSynchronize(ev1));
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
@@ -1104,15 +1129,42 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
                                          " entries, 8 per bank slot); raise bank_capacity");
     // reduce the per-workgroup partial moments (workgroups in order)
     const dim3 rg(256), rb(256);
-    hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.psum, part_rows, (uint64_t)n_bins, t_sum.as<double>());
-    hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.psq, part_rows, (uint64_t)n_bins, t_sq.as<double>());
+    hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, cur, A.psum, part_rows, (uint64_t)n_bins, t_sum.as<double>());
+    hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, cur, A.psq, part_rows, (uint64_t)n_bins, t_sq.as<double>());
     if (grad) {
-        hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.pgsum, part_rows, (uint64_t)n_params,
+        hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, cur, A.pgsum, part_rows, (uint64_t)n_params,
                            t_gsum.as<double>());
-        hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, stream, A.pgsq, part_rows, (uint64_t)n_params,
+        hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, cur, A.pgsq, part_rows, (uint64_t)n_params,
                            t_gsq.as<double>());
     }
-    HIP_CHECK(hipStreamSynchronize(stream));
+    if (dev) {
+        // Estimator::end_cycle / end_simulation (Estimator.cpp:347-367) on the
+        // device, into the caller's device arrays: the same operations as the
+        // host path below, so the same bits
+        std::vector<double> binN(std::max(1, n_bins)), parN(std::max(1, n_params));
+        for (int p = 0; p < np; ++p) {
+            for (int b = prob[p].bin_begin; b < prob[p].bin_begin + prob[p].bin_count; ++b) binN[b] = (double)nh[p];
+            for (int k = prob[p].param_begin; k < prob[p].param_begin + prob[p].param_count; ++k)
+                parN[k] = (double)nh[p];
+        }
+        HIP_CHECK(hipMemcpyAsync(d_binN.p, binN.data(), sizeof(double) * binN.size(), hipMemcpyHostToDevice, cur));
+        HIP_CHECK(hipMemcpyAsync(d_parN.p, parN.data(), sizeof(double) * parN.size(), hipMemcpyHostToDevice, cur));
+        if (out && n_bins > 0)
+            hipLaunchKernelGGL(k_moments, dim3(64), dim3(256), 0, cur, t_sum.as<double>(), t_sq.as<double>(),
+                               d_binN.as<double>(), (uint64_t)n_bins, out->sum, out->sq, out->mean, out->uncer);
+        if (grad && n_params > 0)
+            hipLaunchKernelGGL(k_moments, dim3(64), dim3(256), 0, cur, t_gsum.as<double>(), t_gsq.as<double>(),
+                               d_parN.as<double>(), (uint64_t)n_params, (double*)nullptr, (double*)nullptr,
+                               grad_mean, grad_sigma);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(cur));
+        if (out) {
+            out->flights = flights;
+            out->seconds = ms * 1e-3;
+        }
+        return;
+    }
+    HIP_CHECK(hipStreamSynchronize(cur));
     std::vector<double> sum(n_bins), sq(n_bins);
     HIP_CHECK(hipMemcpy(sum.data(), t_sum.p, sizeof(double) * n_bins, hipMemcpyDeviceToHost));
     HIP_CHECK(hipMemcpy(sq.data(), t_sq.p, sizeof(double) * n_bins, hipMemcpyDeviceToHost));

@@ -573,12 +573,13 @@ __global__ void __launch_bounds__(1024) k_refill_count(KArgs A)
 
 // A new history in slot i: SourceBank::get_source + SourcePoint::get_source
 // (Source.cpp:20-45).  jstate is the LCG multiplier g^(stride * hid) of its
-// key, so the stream starts where RN_init_particle (Random.cpp:196-204) puts it.
+// run key; times run_jd[p] it is g^(stride * key) of the history's key, so
+// the stream starts where RN_init_particle (Random.cpp:196-204) puts it.
 template <bool kGrad>
 __device__ __forceinline__ void start_history(const KArgs& A, uint32_t i, uint64_t hid, int p, uint64_t jstate)
 {
     const DProb PR = A.prob[p];
-    uint64_t rng = (jstate * PR.seed) & kRnMask;
+    uint64_t rng = (((jstate * A.run_jd[p]) & kRnMask) * PR.seed) & kRnMask;
     const double xi = urand(rng);
     int si = binary_search(xi, A.src_cdf + A.cdf_off[p], PR.src_count + 1);
     si = si < 0 ? 0 : (si >= PR.src_count ? PR.src_count - 1 : si);   // cdf[n] may round below 1
@@ -1718,6 +1719,24 @@ __global__ void k_reduce_rows(const double* part, uint64_t rows, uint64_t n, dou
         }
         if (threadIdx.x == 0) out[b] = red[0];
         __syncthreads();
+    }
+}
+
+// Estimator::end_cycle / end_simulation (Estimator.cpp:347-367) per bin or
+// parameter with N histories: mean = sum / N, uncertainty = sqrt((sq / N -
+// mean^2) / (N - 1)); the sums themselves where osum / osq are given.
+__global__ void k_moments(const double* sum, const double* sq, const double* N, uint64_t n, double* osum,
+                          double* osq, double* omean, double* ouncer)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const double s = sum[i], q = sq[i], Nh = N[i];
+        if (osum) osum[i] = s;
+        if (osq) osq[i] = q;
+        const double mean = s / Nh;
+        const double u2 = (q / Nh - mean * mean) / (Nh - 1.0);
+        if (omean) omean[i] = mean;
+        if (ouncer) ouncer[i] = sqrt(u2);
     }
 }
 

@@ -556,4 +556,206 @@ void load_problem_xml(Problem& P, const std::string& xml, const std::string& xs_
     P.finalize();
 }
 
+namespace {
+
+template <class T>
+std::vector<T> copy_n(const T* p, int64_t n, const char* what)
+{
+    if (n > 0 && !p) throw Error(MC_ERR_INPUT, std::string("mc_problem_desc.") + what + " is NULL");
+    return n > 0 ? std::vector<T>(p, p + n) : std::vector<T>();
+}
+
+void in_range(int64_t v, int64_t lo, int64_t hi, const std::string& what)
+{
+    if (v < lo || v >= hi)
+        throw Error(MC_ERR_INPUT, what + " = " + std::to_string(v) + " is outside [" + std::to_string(lo) + ", " +
+                                      std::to_string(hi) + ")");
+}
+
+}  // namespace
+
+void load_problem_desc(Problem& P, const mc_problem_desc& d)
+{
+    need(d.n_surfaces >= 0 && d.n_cells > 0 && d.n_materials >= 0 && d.n_nuclides >= 0 && d.n_sources > 0 &&
+             d.n_estimators >= 0 && d.n_params >= 0 && d.n_bins >= 0,
+         MC_ERR_INPUT, "mc_problem_desc: counts must be non-negative, with at least one cell and one source");
+    P.surfaces = copy_n(d.surfaces, d.n_surfaces, "surfaces");
+    P.cells = copy_n(d.cells, d.n_cells, "cells");
+    P.materials = copy_n(d.materials, d.n_materials, "materials");
+    P.nuclides = copy_n(d.nuclides, d.n_nuclides, "nuclides");
+    P.sources = copy_n(d.sources, d.n_sources, "sources");
+    P.estimators = copy_n(d.estimators, d.n_estimators, "estimators");
+    P.params = copy_n(d.params, d.n_params, "params");
+    // lengths of the indexed arrays follow from the records' ranges
+    int64_t ncs = 0, nmn = 0, nxs = 0, natt = 0, neg = 0, nmp = 0;
+    for (const auto& C : P.cells) {
+        need(C.surf_begin >= 0 && C.surf_count >= 0 && C.tl_begin >= 0 && C.tl_count >= 0 && C.col_begin >= 0 &&
+                 C.col_count >= 0,
+             MC_ERR_INPUT, "mc_problem_desc: negative cell range");
+        ncs = std::max<int64_t>(ncs, (int64_t)C.surf_begin + C.surf_count);
+        natt = std::max<int64_t>(natt, std::max<int64_t>((int64_t)C.tl_begin + C.tl_count,
+                                                         (int64_t)C.col_begin + C.col_count));
+    }
+    for (const auto& S : P.surfaces) {
+        need(S.est_begin >= 0 && S.est_count >= 0, MC_ERR_INPUT, "mc_problem_desc: negative surface range");
+        natt = std::max<int64_t>(natt, (int64_t)S.est_begin + S.est_count);
+    }
+    for (const auto& M : P.materials) {
+        need(M.nuc_begin >= 0 && M.nuc_count >= 0 && M.param_begin >= 0 && M.param_count >= 0, MC_ERR_INPUT,
+             "mc_problem_desc: negative material range");
+        nmn = std::max<int64_t>(nmn, (int64_t)M.nuc_begin + M.nuc_count);
+        nmp = std::max<int64_t>(nmp, (int64_t)M.param_begin + M.param_count);
+    }
+    for (const auto& N : P.nuclides) {
+        if (N.kind != MC_NUC_TABLE) continue;
+        need(N.grid_begin >= 0 && N.grid_count > 0, MC_ERR_INPUT, "mc_problem_desc: empty cross-section table");
+        nxs = std::max<int64_t>(nxs, (int64_t)N.grid_begin + N.grid_count);
+    }
+    for (const auto& E : P.estimators) {
+        need(E.egrid_begin >= 0 && E.egrid_count >= 0, MC_ERR_INPUT, "mc_problem_desc: negative energy-grid range");
+        neg = std::max<int64_t>(neg, (int64_t)E.egrid_begin + E.egrid_count);
+    }
+    P.cell_surf = copy_n(d.cell_surf, ncs, "cell_surf");
+    P.cell_sense = copy_n(d.cell_sense, ncs, "cell_sense");
+    P.mat_nuclide = copy_n(d.mat_nuclide, nmn, "mat_nuclide");
+    P.mat_density = copy_n(d.mat_density, nmn, "mat_density");
+    P.xs_energy = copy_n(d.xs_energy, nxs, "xs_energy");
+    P.xs_scatter = copy_n(d.xs_scatter, nxs, "xs_scatter");
+    P.xs_capture = copy_n(d.xs_capture, nxs, "xs_capture");
+    P.xs_total = copy_n(d.xs_total, nxs, "xs_total");
+    P.att_estimator = copy_n(d.att_estimator, natt, "att_estimator");
+    P.att_index = copy_n(d.att_index, natt, "att_index");
+    P.egrid = copy_n(d.egrid, neg, "egrid");
+    P.mat_param = copy_n(d.mat_param, nmp, "mat_param");
+    // every index in range
+    for (size_t k = 0; k < P.surfaces.size(); ++k) {
+        const auto& S = P.surfaces[k];
+        const bool known = S.type == MC_SURF_PLANE_X || S.type == MC_SURF_PLANE_Y || S.type == MC_SURF_PLANE_Z ||
+                           S.type == MC_SURF_PLANE || S.type == MC_SURF_SPHERE || S.type == MC_SURF_CYL_X ||
+                           S.type == MC_SURF_CYL_Z;
+        need(known, MC_ERR_INPUT, "surface " + std::to_string(k) + ": unknown type " + std::to_string(S.type));
+        need(S.bc >= -1 && S.bc <= 1, MC_ERR_INPUT, "surface " + std::to_string(k) + ": unknown boundary condition");
+    }
+    for (int64_t j = 0; j < ncs; ++j) {
+        in_range(P.cell_surf[j], 0, (int64_t)P.surfaces.size(), "cell_surf[" + std::to_string(j) + "]");
+        need(P.cell_sense[j] == 1 || P.cell_sense[j] == -1, MC_ERR_INPUT,
+             "cell_sense[" + std::to_string(j) + "] must be +1 or -1");
+    }
+    for (size_t c = 0; c < P.cells.size(); ++c) {
+        if (P.cells[c].material != -1)
+            in_range(P.cells[c].material, 0, (int64_t)P.materials.size(), "cells[" + std::to_string(c) + "].material");
+        need(P.cells[c].importance >= 0.0, MC_ERR_INPUT, "cells[" + std::to_string(c) + "]: negative importance");
+    }
+    for (int64_t j = 0; j < nmn; ++j) {
+        in_range(P.mat_nuclide[j], 0, (int64_t)P.nuclides.size(), "mat_nuclide[" + std::to_string(j) + "]");
+        need(P.mat_density[j] >= 0.0, MC_ERR_INPUT, "mat_density[" + std::to_string(j) + "] is negative");
+    }
+    for (size_t k = 0; k < P.nuclides.size(); ++k) {
+        const auto& N = P.nuclides[k];
+        need(N.kind == MC_NUC_CONSTANT || N.kind == MC_NUC_TABLE, MC_ERR_INPUT,
+             "nuclides[" + std::to_string(k) + "]: unknown kind");
+        need(N.A > 0.0, MC_ERR_INPUT, "nuclides[" + std::to_string(k) + "]: non-positive mass");
+        need(N.kinematics == MC_KIN_TARGET_AT_REST || N.kinematics == MC_KIN_FREE_GAS, MC_ERR_INPUT,
+             "nuclides[" + std::to_string(k) + "]: unknown kinematics");
+    }
+    if (nxs > 0) need_finite(P.xs_energy, "mc_problem_desc.xs_energy");
+    int32_t bins = 0;
+    for (size_t e = 0; e < P.estimators.size(); ++e) {
+        const auto& E = P.estimators[e];
+        const std::string en = "estimators[" + std::to_string(e) + "]";
+        need(E.attach_kind == MC_ATTACH_SURFACE || E.attach_kind == MC_ATTACH_CELL, MC_ERR_INPUT,
+             en + ": unknown attachment kind");
+        need(E.n_scores > 0 && E.n_scores <= MC_MAX_SCORES && E.n_attach > 0, MC_ERR_INPUT,
+             en + ": needs 1-8 scores and an attachment");
+        for (int s = 0; s < E.n_scores; ++s) {
+            need(E.score[s] >= MC_SCORE_FLUX && E.score[s] <= MC_SCORE_TOTAL, MC_ERR_INPUT, en + ": unknown score");
+            need(E.score_kernel[s] >= MC_KERNEL_TRACK_LENGTH && E.score_kernel[s] <= MC_KERNEL_NEUTRON, MC_ERR_INPUT,
+                 en + ": unknown score kernel");
+        }
+        need(E.egrid_count == 0 || E.egrid_count >= 2, MC_ERR_INPUT, en + ": an energy filter needs two edges");
+        const int NE = E.egrid_count ? E.egrid_count - 1 : 1;
+        need(E.bin_offset == bins && E.n_bins == E.n_scores * E.n_attach * NE, MC_ERR_INPUT,
+             en + ": bins must follow the [score][attach][energy] layout, estimators in order");
+        bins += E.n_bins;
+    }
+    need(bins == d.n_bins, MC_ERR_INPUT, "mc_problem_desc.n_bins differs from the estimators' bins");
+    P.n_bins = d.n_bins;
+    for (int64_t j = 0; j < natt; ++j) {
+        in_range(P.att_estimator[j], 0, (int64_t)P.estimators.size(), "att_estimator[" + std::to_string(j) + "]");
+        in_range(P.att_index[j], 0, P.estimators[P.att_estimator[j]].n_attach, "att_index[" + std::to_string(j) + "]");
+    }
+    for (size_t k = 0; k < P.params.size(); ++k) {
+        const auto& Q = P.params[k];
+        const std::string pn = "params[" + std::to_string(k) + "]";
+        in_range(Q.cell, 0, (int64_t)P.cells.size(), pn + ".cell");
+        in_range(Q.material, 0, (int64_t)P.materials.size(), pn + ".material");
+        in_range(Q.nuc_slot, 0, P.materials[Q.material].nuc_count, pn + ".nuc_slot");
+        need(Q.reactions >= 1 && Q.reactions <= 3, MC_ERR_INPUT, pn + ": unknown reactions mask");
+        for (size_t o = 0; o < k; ++o)
+            need(!(P.params[o].material == Q.material && P.params[o].nuc_slot == Q.nuc_slot &&
+                   (P.params[o].reactions & Q.reactions)),
+                 MC_ERR_INPUT, pn + " scales a reaction another parameter already scales");
+    }
+    for (int64_t j = 0; j < nmp; ++j) in_range(P.mat_param[j], 0, (int64_t)P.params.size(), "mat_param[" + std::to_string(j) + "]");
+    for (size_t m = 0; m < P.materials.size(); ++m)
+        for (int q = 0; q < P.materials[m].param_count; ++q)
+            need(P.params[P.mat_param[P.materials[m].param_begin + q]].material == (int)m, MC_ERR_INPUT,
+                 "mat_param of material " + std::to_string(m) + " lists a parameter of another material");
+    // sources: the cell of the point (general.cpp:26-34) where not given
+    for (size_t k = 0; k < P.sources.size(); ++k) {
+        auto& S = P.sources[k];
+        need(S.dir_kind == MC_DIR_ISOTROPIC || S.dir_kind == MC_DIR_DELTA, MC_ERR_INPUT,
+             "sources[" + std::to_string(k) + "]: unknown direction kind");
+        need(S.energy_kind == MC_ENERGY_DELTA || S.energy_kind == MC_ENERGY_UNIFORM, MC_ERR_INPUT,
+             "sources[" + std::to_string(k) + "]: unknown energy kind");
+        if (S.cell < 0) S.cell = P.search_cell(S.pos);
+        need(S.cell >= 0, MC_ERR_LOST_PARTICLE, "sources[" + std::to_string(k) + "] is outside every cell");
+        in_range(S.cell, 0, (int64_t)P.cells.size(), "sources[" + std::to_string(k) + "].cell");
+    }
+    if (d.source_cdf) {
+        P.source_cdf.assign(d.source_cdf, d.source_cdf + d.n_sources + 1);
+    } else {
+        // SourceBank::set_up, Source.cpp:65-72
+        P.source_cdf.assign(P.sources.size() + 1, 0.0);
+        const double dp = 1.0 / (double)P.sources.size();
+        for (size_t i = 1; i < P.source_cdf.size(); ++i) P.source_cdf[i] = P.source_cdf[i - 1] + dp;
+    }
+    P.wr = d.wr;
+    P.ws = d.ws;
+    need(P.wr >= 0.0 && P.ws > 0.0, MC_ERR_INPUT, "mc_problem_desc: roulette needs wr >= 0 and ws > 0");
+    P.n_histories = d.n_histories;
+    need(P.n_histories > 1, MC_ERR_INPUT, "mc_problem_desc.n_histories must be at least 2");
+    P.seed = d.seed;
+    need(P.seed > 0 && (P.seed & 1) == 1, MC_ERR_INPUT, "mc_problem_desc.seed must be a positive odd integer");
+    // names by position; report metadata from the records
+    P.name = "problem";
+    for (size_t k = 0; k < P.surfaces.size(); ++k) P.surface_names.push_back("surface " + std::to_string(k));
+    for (size_t k = 0; k < P.cells.size(); ++k) P.cell_names.push_back("cell " + std::to_string(k));
+    for (size_t k = 0; k < P.materials.size(); ++k) P.material_names.push_back("material " + std::to_string(k));
+    for (size_t k = 0; k < P.nuclides.size(); ++k) P.nuclide_names.push_back("nuclide " + std::to_string(k));
+    static const char* kScore[] = {"", "flux", "absorption", "scatter", "capture", "total"};
+    for (size_t e = 0; e < P.estimators.size(); ++e) {
+        const auto& E = P.estimators[e];
+        P.estimator_names.push_back("Estimator #" + std::to_string(e + 1));
+        std::vector<std::string> sn;
+        for (int s = 0; s < E.n_scores; ++s)
+            sn.push_back(E.score_kernel[s] == MC_KERNEL_NEUTRON ? "cross" : kScore[E.score[s]]);
+        P.estimator_scores.push_back(sn);
+        // the filter's geometry IDs, in filter-bin order
+        std::vector<double> ids(E.n_attach, -1.0);
+        auto note = [&](int32_t begin, int32_t count, size_t geom) {
+            for (int32_t a = begin; a < begin + count; ++a)
+                if (P.att_estimator[a] == (int32_t)e) ids[P.att_index[a]] = (double)geom;
+        };
+        for (size_t k = 0; k < P.surfaces.size(); ++k) note(P.surfaces[k].est_begin, P.surfaces[k].est_count, k);
+        for (size_t c = 0; c < P.cells.size(); ++c) {
+            note(P.cells[c].tl_begin, P.cells[c].tl_count, c);
+            note(P.cells[c].col_begin, P.cells[c].col_count, c);
+        }
+        P.estimator_attach_ids.push_back(ids);
+        P.estimator_egrid.emplace_back(P.egrid.begin() + E.egrid_begin, P.egrid.begin() + E.egrid_begin + E.egrid_count);
+    }
+    P.finalize();
+}
+
 }  // namespace mc

@@ -53,6 +53,10 @@ struct Problem {
 // xs_dir: directory of <ZAID>.txt cross-section files.
 void load_problem_xml(Problem& P, const std::string& xml, const std::string& xs_dir);
 
+// Builds a problem from caller-owned flat arrays (mc_problem_create):
+// copies them, checks every index and names the records by position.
+void load_problem_desc(Problem& P, const mc_problem_desc& d);
+
 // Reads one xs_library file (format: xs_library/README.txt:1-7).  Accepts
 // rows of 5 or 6 numbers (the reference reader assumes 6, setup.cpp:366).
 void load_xs_file(const std::string& path, double& A, std::vector<double>& E,

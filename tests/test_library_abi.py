@@ -149,3 +149,76 @@ def test_overlapping_parameters_rejected():
     other = base.replace("</parameters>", '    <parameter cell="bin1" nuclide="onespeed" reaction="total"/>\n'
                                           "</parameters>")
     assert mc.Problem.from_string(other).n_params == 3
+
+
+def _desc_arrays(p):
+    """Raw bytes of every array of a problem's mc_problem_desc."""
+    import numpy as np
+    h = p.head
+    sizes = {"surfaces": (h.n_surfaces, 80), "cells": (h.n_cells, 40), "materials": (h.n_materials, 16),
+             "nuclides": (h.n_nuclides, 48), "sources": (h.n_sources, 88), "estimators": (h.n_estimators, 92),
+             "params": (h.n_params, 16), "source_cdf": (h.n_sources + 1, 8)}
+    out = {}
+    for name, (n, sz) in sizes.items():
+        ptr = getattr(h, name)
+        out[name] = C.string_at(ptr, n * sz) if n and ptr else b""
+    return out
+
+
+@pytest.mark.parametrize("deck", ["slab", "shield24", "box", "reference"])
+def test_problem_create_round_trip(deck):
+    """mc_problem_create on another problem's flat arrays gives the same
+    problem: the same records, and the oracle runs it to the same sums."""
+    import numpy as np
+    from mc_testlib import oracle_run, reference_shielding_deck
+    if deck == "slab":
+        p = mc.Problem.from_string(decks.slab_deck(3.0, 0.7, 10, 500, params=("scatter", "capture")))
+    elif deck == "shield24":
+        p = mc.Problem.from_string(decks.shield_deck(500, params="24"), shield_xs_dir())
+    elif deck == "box":
+        p = mc.Problem.from_string(decks.box_deck(500), shield_xs_dir())
+    else:
+        p = reference_shielding_deck()
+    q = mc.Problem.from_desc(p.desc_ptr)
+    assert _desc_arrays(p) == _desc_arrays(q)
+    for f in ("n_bins", "n_params", "n_histories", "seed", "wr", "ws"):
+        assert getattr(p.head, f) == getattr(q.head, f), f
+    go = np.ones(p.n_bins) if p.n_params else None
+    a, b = oracle_run(p, 300, grad_out=go), oracle_run(q, 300, grad_out=go)
+    assert (a.sum == b.sum).all() and (a.sq == b.sq).all() and a.flights == b.flights
+    assert (a.gsum == b.gsum).all()
+    assert q.estimator_names()[0] == "Estimator #1"
+
+
+def test_problem_create_checks_indices():
+    import numpy as np
+    p = mc.Problem.from_string(decks.slab_deck(2.0, 0.5, 4, 100))
+    h = p.head
+
+    def with_array(field, dtype, n, patch):
+        arr = np.ctypeslib.as_array((C.c_byte * (n * np.dtype(dtype).itemsize)).from_address(getattr(h, field)))
+        arr = arr.view(dtype).copy()
+        patch(arr)
+        d = mc.ProblemDesc.from_buffer_copy(h)
+        setattr(d, field, arr.ctypes.data)
+        out = C.c_void_p()
+        st = mc.lib.mc_problem_create(C.byref(d), C.byref(out))
+        if st == 0:
+            mc.lib.mc_problem_destroy(out)
+        return st, mc.lib.mc_last_error().decode()
+
+    ncs = 2 * h.n_cells
+    st, msg = with_array("cell_surf", np.int32, ncs, lambda a: a.__setitem__(3, 999))
+    assert st == mc.MC_ERR_INPUT and "cell_surf[3]" in msg
+    st, msg = with_array("cell_sense", np.int32, ncs, lambda a: a.__setitem__(0, 2))
+    assert st == mc.MC_ERR_INPUT and "cell_sense[0]" in msg
+    st, msg = with_array("mat_nuclide", np.int32, 1, lambda a: a.__setitem__(0, 5))
+    assert st == mc.MC_ERR_INPUT and "mat_nuclide[0]" in msg
+    d = mc.ProblemDesc.from_buffer_copy(h)
+    d.n_bins = h.n_bins + 1
+    out = C.c_void_p()
+    assert mc.lib.mc_problem_create(C.byref(d), C.byref(out)) == mc.MC_ERR_INPUT
+    d = mc.ProblemDesc.from_buffer_copy(h)
+    d.seed = 2
+    assert mc.lib.mc_problem_create(C.byref(d), C.byref(out)) == mc.MC_ERR_INPUT
+    assert mc.lib.mc_problem_create(None, C.byref(out)) == mc.MC_ERR_ARGUMENT
