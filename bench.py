@@ -8,15 +8,18 @@ with cross-section tables and problem data resident in HBM before timing.
 
 Also reported on the same JSON line:
   batched_vs_plain  the 4096-slab sweep in one batched launch sequence vs one
-                    problem per launch sequence (plain timed on a seeded subset)
+                    problem per launch sequence (both over all 4096 problems)
   bwd_fwd           likelihood-ratio backward cost / forward cost, 24-parameter shield
   box_variants      64 design variants of the 3D box shield in one batched engine:
                     detector response and its 80-parameter gradient per variant
-                    (--box-histories total, 1e8 by default; the full workload is 1e9)
+                    (--box-histories total, 1e9 by default: the full workload)
   xs_lookup         the standalone lookup kernel against HBM peak
-  roofline          the dominant event-loop kernel (algorithmic bytes per particle
-                    event, device time from per-kernel HIP events)
-  tally_check       GPU tallies vs the CPU oracle sample (combined-sigma z scores)
+  roofline          the dominant event-loop kernel: algorithmic bytes per event
+                    (one event = one flight, 152 B in continuous energy) x the
+                    events of a launch / its device time from per-kernel HIP
+                    events; the per-particle figure of the fused kernel beside it
+  tally_check       GPU tallies vs the CPU oracle's histories (the single-core
+                    sample and the parallel workers' pooled; combined-sigma z scores)
   cpu_baseline      the oracle (C restatement, 1 core) on a bounded sample
   cpu_baseline_parallel
                     the same oracle as independent processes over disjoint
@@ -43,9 +46,17 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, AMD public spec
 
-# Algorithmic HBM bytes per particle event for each event kernel (particle
-# state read + written, queue entries); tables and geometry are cache-resident
-# and not counted.
+# Algorithmic HBM bytes per event of the event loop (an event is one flight,
+# as the reference counts Ntrack, general.cpp:76): the particle state read and
+# written once per event, 2 x (position 24 + direction 24 + E 8 + weight 8 +
+# cell 4 + rng 8) = 152 B in continuous energy.  The fused flight kernel
+# reads and writes a particle once per launch however many flights it takes,
+# so its own bytes per particle are below (ALGO_BYTES["k_flight"]).
+EVENT_BYTES = 152
+
+# Algorithmic HBM bytes per item of each event-loop kernel (particle state
+# read + written, queue entries); tables and geometry are cache-resident and
+# not counted.
 ALGO_BYTES = {
     "k_xs_lookup": 28,       # queue 4 + material 4 + E 8 in; Sigma_t 8 + grid interval 4 out
     "k_flight": 189,         # forward runs, per PARTICLE per launch: the particle's state is read once,
@@ -82,16 +93,19 @@ def cpu_baseline(mc, decks, xs, n_sample):
     return r, dt
 
 
-def cpu_worker(offset, n, xs):
+def cpu_worker(offset, n, xs, out_path):
     """One process of the parallel CPU baseline: n oracle histories of the
-    shield from key offset on (no GPU use)."""
+    shield from key offset on (no GPU use); their tally moments go to
+    out_path (.npy) for the tally check."""
     from mc_testlib import oracle_run
     mc = importlib.import_module("mc-old_amd")
     decks = importlib.import_module("mc-old_amd.decks")
     p = mc.Problem.from_string(decks.shield_deck(n), xs)
     t = time.perf_counter()
-    oracle_run(p, n, key_offset=offset)
-    print(json.dumps({"seconds": time.perf_counter() - t, "histories": n}), flush=True)
+    r = oracle_run(p, n, key_offset=offset)
+    dt = time.perf_counter() - t
+    np.save(out_path, np.stack([r.sum, r.sq]))
+    print(json.dumps({"seconds": dt, "histories": n}), flush=True)
 
 
 def cpu_parallel(xs, n_each):
@@ -104,22 +118,30 @@ def cpu_parallel(xs, n_each):
     except AttributeError:
         cores = os.cpu_count() or 1
     P = max(1, min(16, cores))
+    import tempfile
+    tmp = tempfile.mkdtemp(prefix="mc_cpu_")
+    paths = [os.path.join(tmp, f"w{k}.npy") for k in range(P)]
     t = time.perf_counter()
     procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
-                               str(2 * 10 ** 12 + k * n_each), str(n_each), xs],
+                               str(2 * 10 ** 12 + k * n_each), str(n_each), xs, paths[k]],
                               stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, cwd=ROOT)
              for k in range(P)]
-    outs = [pr.communicate(timeout=600) for pr in procs]
+    outs = [pr.communicate(timeout=900) for pr in procs]
     wall = time.perf_counter() - t
     if any(pr.returncode != 0 for pr in procs):
         raise RuntimeError("cpu worker failed: " + " | ".join(o[1][-300:] for o in outs))
     # each process times its own histories (interpreter start-up excluded);
     # they run side by side, so the slowest one bounds the aggregate
     secs = [json.loads([l for l in o[0].splitlines() if l.startswith("{")][-1])["seconds"] for o in outs]
+    moments = sum(np.load(p) for p in paths)
+    for p in paths:
+        os.remove(p)
+    os.rmdir(tmp)
     return {"value": P * n_each / max(secs), "unit": "histories/s", "cores": P, "kind": "port",
             "sample": f"{P} processes x {n_each} histories of the same shield deck on disjoint keys "
                       f"(oracle C restatement), slowest {max(secs):.1f} s ({wall:.1f} s wall with start-up), "
-                      f"{platform.processor() or platform.machine()}, nproc={os.cpu_count()}"}
+                      f"{platform.processor() or platform.machine()}, nproc={os.cpu_count()}"}, \
+        (P * n_each, moments)
 
 
 def zscores(gm, gu, om, ou, min_rel=0.2):
@@ -220,7 +242,7 @@ def kernel_roofline(mc, prob, H, traffic):
     dom = max((k for k in table if k in ALGO_BYTES), key=lambda k: table[k]["seconds"])
     d = table[dom]
     epl = d["items"] / d["launches"]
-    upl = d.get("particles", d["items"]) / d["launches"]     # units the algorithmic bytes count
+    upl = d.get("particles", d["items"]) / d["launches"]
     tr = None
     if traffic and dom in traffic["bytes_per_item"]:
         t = traffic["bytes_per_item"][dom]
@@ -228,11 +250,23 @@ def kernel_roofline(mc, prob, H, traffic):
         for k, row in table.items():
             if k in traffic["bytes_per_item"]:
                 row["hbm_B_per_item"] = traffic["bytes_per_item"][k]
-    roof = {"bound": "hbm", "kernel": dom, "achieved": d["algo_GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": tr,
-            "bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": upl,
-            "unit_of_bytes": "particle" if dom == "k_flight" else "event",
-            "avg_launch_us": d["avg_us"], "events_per_launch": epl}
+    avg_s = d["seconds"] / d["launches"]
+    if dom == "k_flight":
+        # per event (one event = one flight): EVENT_BYTES x the launch's flights
+        achieved = EVENT_BYTES * epl / avg_s / 1e9
+        per_particle = {"bytes_per_particle": ALGO_BYTES[dom], "particles_per_launch": upl,
+                        "achieved": d["algo_GBs"], "frac": d["algo_GBs"] / HBM_PEAK_GBS,
+                        "note": "the fused kernel reads and writes each particle once per launch"}
+        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": tr, "bytes_per_unit": EVENT_BYTES,
+                "units_per_launch": epl, "unit_of_bytes": "event (flight)", "avg_launch_us": d["avg_us"],
+                "frac_per_event": achieved / HBM_PEAK_GBS, "per_particle": per_particle,
+                "traffic_per_event": tr / epl if tr else None}
+    else:
+        roof = {"bound": "hbm", "kernel": dom, "achieved": d["algo_GBs"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": d["algo_GBs"] / HBM_PEAK_GBS, "traffic": tr,
+                "bytes_per_unit": ALGO_BYTES[dom], "units_per_launch": upl, "unit_of_bytes": "event",
+                "avg_launch_us": d["avg_us"]}
     if traffic:
         roof["traffic_source"] = ("rocprofv3 FETCH_SIZE / WRITE_SIZE over the traffic probe "
                                   f"({traffic['histories']} histories, plain launches), scaled by a "
@@ -249,8 +283,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories", type=float, default=1e8)
     ap.add_argument("--cpu-sample", type=int, default=100000)
-    ap.add_argument("--box-histories", type=float, default=1e8,
-                    help="total histories of the 64-variant 3D box run (the full workload is 1e9)")
+    ap.add_argument("--cpu-parallel-sample", type=int, default=200000,
+                    help="histories per parallel CPU worker (pooled into the tally check)")
+    ap.add_argument("--box-histories", type=float, default=1e9,
+                    help="total histories of the 64-variant 3D box run (the full workload: 1e9)")
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
@@ -258,10 +294,10 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the headline deck's per-kernel profile run and its roofline line "
                          "(for profiling the dominant kernel)")
-    ap.add_argument("--cpu-worker", nargs=3, metavar=("OFFSET", "N", "XS"), help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-worker", nargs=4, metavar=("OFFSET", "N", "XS", "OUT"), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
-        cpu_worker(int(args.cpu_worker[0]), int(args.cpu_worker[1]), args.cpu_worker[2])
+        cpu_worker(int(args.cpu_worker[0]), int(args.cpu_worker[1]), args.cpu_worker[2], args.cpu_worker[3])
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
@@ -284,11 +320,11 @@ def main():
         except Exception as ex:      # the counters are a report, not a gate
             log(f"traffic passes skipped: {ex}")
 
-    cpu_par = None
+    cpu_par, pooled = None, None
     if not (args.quick or args.roofline_only):
         # child processes too, so also before this process uses the GPU
         try:
-            cpu_par = cpu_parallel(shield_xs_dir(), max(1000, args.cpu_sample // 2))
+            cpu_par, pooled = cpu_parallel(shield_xs_dir(), max(1000, args.cpu_parallel_sample))
             log(f"cpu oracle, {cpu_par['cores']} processes: {cpu_par['value']:.4g} hist/s")
         except Exception as ex:      # a reported baseline, not a gate
             log(f"parallel cpu baseline skipped: {ex}")
@@ -345,10 +381,21 @@ def main():
     # ------------------------------------------------- tally agreement + CPU
     n_cpu = args.cpu_sample
     r_cpu, dt_cpu = cpu_baseline(mc, decks, xs, n_cpu)
-    z = zscores(res.mean, res.uncer, r_cpu.mean, r_cpu.uncer)
-    out["tally_check"] = {"reference": f"oracle, {n_cpu} histories (disjoint keys)", "bins": int(z.size),
+    # the oracle's histories: this process's sample plus the parallel workers'
+    # (all on keys disjoint from the GPU run's and from each other)
+    n_or, s_or, q_or = n_cpu, r_cpu.sum.copy(), r_cpu.sq.copy()
+    if pooled is not None:
+        n_or += pooled[0]
+        s_or += pooled[1][0]
+        q_or += pooled[1][1]
+    om = s_or / n_or
+    ou = np.sqrt(np.maximum(q_or / n_or - om * om, 0.0) / (n_or - 1.0))
+    z = zscores(res.mean, res.uncer, om, ou)
+    out["tally_check"] = {"reference": f"oracle, {n_or} histories on keys disjoint from the GPU run's",
+                          "bins": int(z.size), "bins_total": int(res.mean.size),
                           "max_abs_z": float(np.abs(z).max()) if z.size else 0.0,
-                          "frac_abs_z_gt_3": float(np.mean(np.abs(z) > 3)) if z.size else 0.0}
+                          "frac_abs_z_gt_3": float(np.mean(np.abs(z) > 3)) if z.size else 0.0,
+                          "bins_resolved": "bins whose oracle estimate has relative error <= 0.2"}
     out["cpu_baseline"] = {"value": n_cpu / dt_cpu, "unit": "histories/s", "cores": 1, "kind": "port",
                            "sample": f"{n_cpu} histories of the same shield deck (oracle C restatement, "
                                      f"{dt_cpu:.1f} s, {platform.processor() or platform.machine()}, "
@@ -383,10 +430,9 @@ def main():
         br = beng.forward()
         tb = time.perf_counter() - t
         rate_b = float(N.sum()) / tb
-        rng = np.random.default_rng(11)
-        sub = np.sort(rng.choice(4096, 64, replace=False))
+        sub = np.arange(4096)
         # plain: the same kernels and launch shape, one problem at a time with
-        # ordinary launches (no graph replay, no batching)
+        # ordinary launches (no graph replay, no batching), over all 4096 problems
         peng2 = mc.Engine([probs[i] for i in sub], use_graph=False, block=tune.block, grid_per_cu=tune.grid_per_cu,
                           bank_capacity=int(tune.bank_capacity), drain_divisor=tune.drain_divisor,
                           events_per_launch=tune.events_per_launch)
@@ -408,7 +454,7 @@ def main():
         rate_pg = plain_rate(use_graph=G, drain_divisor=tune.drain_divisor)
         rate_p3 = plain_rate(use_graph=False, drain_divisor=-1)
         out["batched_vs_plain"] = {"problems": 4096, "histories": int(N.sum()), "batched_seconds": tb,
-                                   "batched_hist_per_s": rate_b, "plain_subset": int(sub.size),
+                                   "batched_hist_per_s": rate_b, "plain_problems": int(sub.size),
                                    "plain_seconds": tp, "plain_hist_per_s": rate_p, "speedup": rate_b / rate_p,
                                    "flights": int(br.flights), "autotuned": tuned,
                                    "plain_definition": "same kernels and launch shape, one problem at a time, "

@@ -20,7 +20,8 @@
  *   mc_backward                (no counterpart: likelihood-ratio gradient of the tallies)
  *   mc_xs_lookup               Material::SigmaT(E)               src/Material.cpp:50-57
  *   mc_autotune                (no counterpart: launch-shape search for the batched loop)
- *   mc_report_text             Simulator::report(io_dir)         src/simulator/report.cpp:9-50
+ *   mc_report_h5               Simulator::report(io_dir)         src/simulator/report.cpp:9-50
+ *   mc_report_text               (the same content as text, where HDF5 is absent)
  *
  * Ownership: the library owns mc_problem / mc_engine handles; the caller owns
  * every output buffer.  An engine copies what it needs from its problems when
@@ -190,6 +191,12 @@ mc_status mc_problem_create(const mc_problem_desc* desc, mc_problem** out);
 mc_status mc_problem_get_desc(const mc_problem* p, const mc_problem_desc** out);
 /* Estimator name / score name lookups for reports (NULL when out of range). */
 const char* mc_problem_estimator_name(const mc_problem* p, int32_t e);
+/* Deck name of surface, cell, material (a per-cell copy made for gradient
+ * parameters is "<material>@<cell>"), nuclide or estimator i (NULL when out
+ * of range): the names the reference's objects carry (Geometry::name,
+ * Material::name, Nuclide::name, Estimator::name).                          */
+enum { MC_OBJ_SURFACE = 1, MC_OBJ_CELL = 2, MC_OBJ_MATERIAL = 3, MC_OBJ_NUCLIDE = 4, MC_OBJ_ESTIMATOR = 5 };
+const char* mc_problem_object_name(const mc_problem* p, int32_t kind, int32_t i);
 void mc_problem_destroy(mc_problem* p);
 
 /* ------------------------------------------------------------------- engine */
@@ -318,6 +325,15 @@ typedef struct mc_tune_result {
 mc_status mc_autotune(mc_engine* e, uint64_t sample_histories, mc_tune_result* out);
 
 void mc_engine_destroy(mc_engine* e);
+
+/* output.h5 of Simulator::report (report.cpp:9-50, Estimator.cpp:368-422):
+ * /summary/{Ncycle,Nsample,Npassive,Ntrack,mode}, /summary/survival_roulette/
+ * {wr,ws}, and per estimator /<name>/ (attribute "indexing", one dataset per
+ * filter grid with attribute "unit", /<name>/<score>/{mean,uncertainty} shaped
+ * by the filter sizes).  Written with the HDF5 C library loaded at run time
+ * ($MC_HDF5_LIBRARY, libhdf5.so, /opt/conda/lib/libhdf5.so): MC_ERR_UNSUPPORTED
+ * when none loads (mc_report_text writes the same content as text).          */
+mc_status mc_report_h5(const mc_problem* p, const mc_tally_out* t, uint64_t n_histories, const char* path);
 
 /* Text report in the output.h5 layout of report.cpp:9-50 / Estimator.cpp:368-422. */
 mc_status mc_report_text(const mc_problem* p, const mc_tally_out* t, uint64_t n_histories,

@@ -105,6 +105,7 @@ EXPORTS = {
     "mc_problem_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mc_problem_get_desc": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "mc_problem_estimator_name": (C.c_char_p, [C.c_void_p, C.c_int32]),
+    "mc_problem_object_name": (C.c_char_p, [C.c_void_p, C.c_int32, C.c_int32]),
     "mc_problem_destroy": (None, [C.c_void_p]),
     "mc_engine_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(EngineCfg), C.POINTER(C.c_void_p)]),
     "mc_engine_info_get": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),
@@ -122,6 +123,7 @@ EXPORTS = {
     "mc_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(TuneResult)]),
     "mc_engine_destroy": (None, [C.c_void_p]),
     "mc_report_text": (C.c_int, [C.c_void_p, C.POINTER(TallyOut), C.c_uint64, C.c_char_p]),
+    "mc_report_h5": (C.c_int, [C.c_void_p, C.POINTER(TallyOut), C.c_uint64, C.c_char_p]),
 }
 
 
@@ -431,13 +433,19 @@ class Simulator:
             off += n
 
     def report(self, io_dir: str | None = None) -> str:
+        """<io_dir>/output.h5 in the reference's layout (report.cpp:9-50), or
+        output.txt with the same content where no HDF5 library loads."""
         if self._result is None:
             raise RuntimeError("start() has not run")
         d = io_dir or self.io_dir
-        path = os.path.join(d, "output.txt")
         r = self._result
         out = TallyOut(_dptr(r.sum), _dptr(r.sq), _dptr(r.mean), _dptr(r.uncer), r.flights, r.seconds)
-        _check(lib.mc_report_text(self.problem._h, C.byref(out), int(self.Nsample), path.encode()))
+        path = os.path.join(d, "output.h5")
+        st = lib.mc_report_h5(self.problem._h, C.byref(out), int(self.Nsample), path.encode())
+        if st == MC_ERR_UNSUPPORTED:
+            path = os.path.join(d, "output.txt")
+            st = lib.mc_report_text(self.problem._h, C.byref(out), int(self.Nsample), path.encode())
+        _check(st)
         return path
 
 

@@ -19,8 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HOST = os.path.join(HERE, "host")
 CLI = os.path.join(HERE, "mc")
-SOURCES = ["csrc/mc_xml.cpp", "csrc/mc_problem.cpp", "csrc/mc_api.cpp", "csrc/mc_engine.hip",
-           "csrc/mc_kernels.hip", "host/simulator.cpp"]
+SOURCES = ["csrc/mc_xml.cpp", "csrc/mc_problem.cpp", "csrc/mc_api.cpp", "csrc/mc_report.cpp",
+           "csrc/mc_engine.hip", "csrc/mc_kernels.hip", "host/simulator.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
           "-Wno-unused-function", "-I" + CSRC]
 DEVICE = ["--offload-arch=gfx950", "-munsafe-fp-atomics",
@@ -45,7 +45,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs]
+    cmd = [HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs, "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -105,6 +105,20 @@ const char* mc_problem_estimator_name(const mc_problem* p, int32_t e)
     return p->P.estimator_names[e].c_str();
 }
 
+const char* mc_problem_object_name(const mc_problem* p, int32_t kind, int32_t i)
+{
+    if (!p || i < 0) return nullptr;
+    const mc::Problem& P = p->P;
+    const std::vector<std::string>* v = kind == MC_OBJ_SURFACE     ? &P.surface_names
+                                        : kind == MC_OBJ_CELL      ? &P.cell_names
+                                        : kind == MC_OBJ_MATERIAL  ? &P.material_names
+                                        : kind == MC_OBJ_NUCLIDE   ? &P.nuclide_names
+                                        : kind == MC_OBJ_ESTIMATOR ? &P.estimator_names
+                                                                   : nullptr;
+    if (!v || i >= (int32_t)v->size()) return nullptr;
+    return (*v)[i].c_str();
+}
+
 void mc_problem_destroy(mc_problem* p) { delete p; }
 
 mc_status mc_engine_create(mc_problem* const* problems, int32_t n, const mc_engine_cfg* cfg, mc_engine** out)
@@ -235,6 +249,14 @@ mc_status mc_report_text(const mc_problem* p, const mc_tally_out* t, uint64_t n_
                 o << "\n";
             }
         }
+    });
+}
+
+mc_status mc_report_h5(const mc_problem* p, const mc_tally_out* t, uint64_t n_histories, const char* path)
+{
+    return guarded([&] {
+        if (!p || !t || !path || !t->mean || !t->uncer) throw mc::Error(MC_ERR_ARGUMENT, "null argument");
+        mc::report_h5(p->P, *t, n_histories, path);
     });
 }
 

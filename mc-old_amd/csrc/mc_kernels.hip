@@ -86,6 +86,15 @@ __device__ __forceinline__ void wg_prefix(const uint32_t* cnt, uint32_t& a_befor
     b_tot = bt;
 }
 
+// The same value, hidden from the optimiser: addresses derived from it are
+// computed afresh after this point instead of being kept live (in registers,
+// or spilled) from an earlier computation.
+__device__ __forceinline__ uint32_t opaque(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 #define MC_GRID_LOOP(count)                                                                \
     for (uint32_t _i = blockIdx.x * blockDim.x + threadIdx.x; _i < (count); _i += gridDim.x * blockDim.x)
 
@@ -1251,10 +1260,13 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                 PState R = load_state(A, s, kTables);
                 const int m = kTables ? A.pmat[s] : -1;
                 for (;;) {
-                    st = flight_one<kTables, kGrad, true, true>(A, s, crossed, &R);
+                    // the slot's field addresses are recomputed where they are
+                    // used (one add each) rather than held across the events
+                    const uint32_t se = opaque(s);
+                    st = flight_one<kTables, kGrad, true, true>(A, se, crossed, &R);
                     ++nfl;
                     if (st != ST_COLL) break;
-                    st = collision_one<kTables, kGrad, false, true>(A, s, &R);
+                    st = collision_one<kTables, kGrad, false, true>(A, se, &R);
                     if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
                     // once fewer than a quarter of the wave's lanes are still
                     // flying, the rest stop too and fly on next iteration (the
@@ -1268,8 +1280,9 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                         R.gidx = g;
                     }
                 }
-                store_state(A, s, R, kTables);
-                A.status[s] = st;
+                const uint32_t so = opaque(s);
+                store_state(A, so, R, kTables);
+                A.status[so] = st;
             } else {
                 // one flight; its collision goes to queue C
                 st = flight_one<kTables, kGrad, true>(A, s, crossed);
@@ -1612,8 +1625,9 @@ __global__ void __launch_bounds__(1024) k_drain(KArgs A, uint32_t max_events)
     uint32_t nfl = 0, nsurf = 0, ncoll = 0;
     MC_GRID_LOOP(n)
     {
-        const uint32_t s = qa_at(A.qA, A.ctr->qa_ident, _i);
+        const uint32_t s0 = qa_at(A.qA, A.ctr->qa_ident, _i);
         for (uint32_t e = 0; e < max_events; ++e) {
+            const uint32_t s = opaque(s0);     // field addresses recomputed per event
             if (kTables) {
                 const int m = A.pmat[s];
                 if (m >= 0) {

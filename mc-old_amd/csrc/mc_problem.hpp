@@ -62,6 +62,12 @@ void load_problem_desc(Problem& P, const mc_problem_desc& d);
 void load_xs_file(const std::string& path, double& A, std::vector<double>& E,
                   std::vector<double>& s, std::vector<double>& c, std::vector<double>& f);
 
+// output.h5 in the reference's layout (report.cpp:9-50, Estimator.cpp:368-422)
+// through the HDF5 C library loaded at run time (mc_report.cpp); throws
+// MC_ERR_UNSUPPORTED when none can be loaded.
+bool hdf5_available();
+void report_h5(const Problem& P, const mc_tally_out& t, uint64_t n_histories, const std::string& path);
+
 // Reference helpers reproduced for setup-time arithmetic.
 int ref_binary_search(double x, const double* v, int n);   // Algorithm.cpp:46-64
 

@@ -178,10 +178,11 @@ def _shield_common(samples: float, seed: int) -> tuple[str, str]:
     return head, ""
 
 
-def shield_deck(samples: float = 1e8, seed: int = 1, params: str = "", leakage: bool = True) -> str:
+def shield_deck(samples: float = 1e8, seed: int = 1, params: str = "", leakage: bool = True,
+                transmission: bool = True) -> str:
     """Three-layer CE shield.  params: "" or "24" (3 layers x 4 nuclides x
     {scatter, capture} scale factors).  leakage=False keeps only the
-    transmission estimator."""
+    transmission estimator, transmission=False only the leakage one."""
     head, _ = _shield_common(samples, seed)
     planes = [0.0, 20.0, 30.0, 60.0]
     surfs = "\n".join(f'    <plane_x name="px{k}" x="{_f(x)}"/>' for k, x in enumerate(planes))
@@ -210,12 +211,14 @@ def shield_deck(samples: float = 1e8, seed: int = 1, params: str = "", leakage: 
     <point x="1e-9" y="0.0" z="0.0"/>
 </sources>
 <estimators>
-    <estimator name="transmission" scores="cross">
-        <surface name="px3"/>
-    </estimator>
-{LEAKAGE if leakage else ""}</estimators>
+{TRANSMISSION if transmission else ""}{LEAKAGE if leakage else ""}</estimators>
 {prm}"""
 
+
+TRANSMISSION = """    <estimator name="transmission" scores="cross">
+        <surface name="px3"/>
+    </estimator>
+"""
 
 LEAKAGE = """    <estimator name="leakage" scores="cross">
         <surface name="px0"/>

@@ -1,7 +1,8 @@
 // main.cpp — `mc <io_dir> [xs_dir]`: the drop-in for the reference's
 // `MC.exe <io_dir>` (Main.cpp:8-31).  Reads <io_dir>/input.xml, runs the
-// fixed-source simulation on the GPU and writes <io_dir>/output.txt in the
-// output.h5 layout (report.cpp:9-50).
+// fixed-source simulation on the GPU and writes <io_dir>/output.h5 in the
+// reference's layout (report.cpp:9-50), or output.txt with the same content
+// where no HDF5 library loads.
 #include <cstdio>
 #include <string>
 
@@ -22,8 +23,8 @@ int main(int argc, char** argv)
         std::printf("Simulation: %llu histories\n", S.Nsample);
         S.start();
         S.report(io_dir);
-        std::printf("Done: %llu flights in %.3f s of device time; wrote %soutput.txt\n", S.Ntrack,
-                    S.device_seconds, io_dir.c_str());
+        std::printf("Done: %llu flights in %.3f s of device time; wrote %s\n", S.Ntrack, S.device_seconds,
+                    S.report_path.c_str());
         for (const auto& E : S.Estimators)
             if (E->tally_size() > 0)
                 std::printf("  %-24s bin 0: %.6g +- %.3g\n", E->name().c_str(), E->tally(0).mean,

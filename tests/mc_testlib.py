@@ -405,3 +405,73 @@ def quadric_deck(n: int) -> str:
     </estimator>
 </estimators>
 """
+
+
+class HipRuntime:
+    """The HIP runtime libmcold.so itself links (/opt/rocm), through ctypes:
+    device buffers and streams for the device-pointer entry points.  (torch
+    bundles a HIP runtime of its own; two runtimes in one process do not
+    share the device, so tests take device memory from the library's.)"""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self):
+        path = None
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64.so" in line and "torch" not in line:
+                    path = line.split()[-1]
+                    break
+        self.lib = C.CDLL(path or "/opt/rocm/lib/libamdhip64.so")
+        L = self.lib
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipFree.argtypes = [C.c_void_p]
+        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        L.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+        L.hipStreamDestroy.argtypes = [C.c_void_p]
+
+    def _ok(self, st, what):
+        if st != 0:
+            raise RuntimeError(f"{what} failed with HIP error {st}")
+
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        self._ok(self.lib.hipMalloc(C.byref(p), nbytes), "hipMalloc")
+        return p.value
+
+    def free(self, p):
+        self._ok(self.lib.hipFree(C.c_void_p(p)), "hipFree")
+
+    def upload(self, arr):
+        p = self.malloc(arr.nbytes)
+        self._ok(self.lib.hipMemcpy(C.c_void_p(p), arr.ctypes.data, arr.nbytes, self.H2D), "hipMemcpy")
+        return p
+
+    def download(self, p, arr):
+        self._ok(self.lib.hipMemcpy(arr.ctypes.data, C.c_void_p(p), arr.nbytes, self.D2H), "hipMemcpy")
+        return arr
+
+    def fill(self, p, byte, nbytes):
+        self._ok(self.lib.hipMemset(C.c_void_p(p), byte, nbytes), "hipMemset")
+
+    def stream(self):
+        s = C.c_void_p()
+        self._ok(self.lib.hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        return s.value
+
+    def stream_destroy(self, s):
+        self._ok(self.lib.hipStreamDestroy(C.c_void_p(s)), "hipStreamDestroy")
+
+
+def with_roulette(problem, wr: float, ws: float = 1.0):
+    """The same problem built by mc_problem_create with weight-roulette
+    threshold wr and survival weight ws (simulator.h:90-91 fixes 0.001 and
+    1; wr = 0 switches roulette off)."""
+    d = mc.ProblemDesc.from_buffer_copy(problem.head)
+    d.wr, d.ws = wr, ws
+    h = C.c_void_p()
+    st = mc.lib.mc_problem_create(C.byref(d), C.byref(h))
+    if st != 0:
+        raise mc.MCError(st, mc.lib.mc_last_error().decode())
+    return mc.Problem(h.value)

@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from mc_testlib import decks, mc, oracle_run, shield_xs_dir
+from mc_testlib import HipRuntime, decks, mc, oracle_run, shield_xs_dir
 
 pytestmark = pytest.mark.gpu
 
@@ -66,37 +66,49 @@ def test_batched_key_offsets_match_oracle():
 
 
 def test_device_pointers_on_a_caller_stream():
-    """Outputs written to torch device tensors by a run issued on a torch
-    stream: the same values, bit for bit, as the host-pointer run; gradient
-    weights read from and gradients written to device memory likewise."""
-    torch = pytest.importorskip("torch")
+    """Outputs written to device buffers by a run issued on a caller's HIP
+    stream: the same records, bit for bit, and the same moments to rounding
+    as the host-pointer run; gradient weights read from and gradients
+    written to device memory likewise."""
     n = 8000
     p = mc.Problem.from_string(decks.shield_deck(n, params="24"), shield_xs_dir())
     eng = mc.Engine(p, bank_capacity=1 << 12)
     host = eng.forward(n_histories=n, trace=n)
-    nb = p.n_bins
-    dev = torch.device("cuda:0")
-    s, q, m, u = (torch.full((nb,), -7.0, dtype=torch.float64, device=dev) for _ in range(4))
-    rec = torch.zeros(n * mc.TRACE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
-    st = torch.cuda.Stream(device=dev)
-    torch.cuda.synchronize()               # the tensors' fills are done before the run writes them
-    r = eng.forward_device(s.data_ptr(), q.data_ptr(), m.data_ptr(), u.data_ptr(), n_histories=n,
-                           stream=st.cuda_stream, trace_ptr=rec.data_ptr(), trace=n)
-    torch.cuda.synchronize()
-    assert r["flights"] == host.flights
-    assert (s.cpu().numpy() == host.sum).all() and (q.cpu().numpy() == host.sq).all()
-    assert (m.cpu().numpy() == host.mean).all() and (u.cpu().numpy() == host.uncer).all()
-    same_records(np.frombuffer(rec.cpu().numpy().tobytes(), dtype=mc.TRACE_DTYPE), host.trace, "device trace")
-    # backward with device-side weights and outputs
-    go = np.random.default_rng(9).normal(size=nb)
-    hb = eng.backward(go, n_histories=n)
-    gd = torch.from_numpy(go).to(dev)
-    g, gs = (torch.zeros(p.n_params, dtype=torch.float64, device=dev) for _ in range(2))
-    torch.cuda.synchronize()
-    eng.backward_device(gd.data_ptr(), g.data_ptr(), gs.data_ptr(), n_histories=n, stream=st.cuda_stream)
-    torch.cuda.synchronize()
-    np.testing.assert_allclose(g.cpu().numpy(), hb.grad, rtol=1e-10, atol=1e-14)
-    np.testing.assert_allclose(gs.cpu().numpy(), hb.grad_sigma, rtol=1e-8)
+    hip = HipRuntime()
+    nb, npar = p.n_bins, p.n_params
+    bufs = [hip.malloc(8 * nb) for _ in range(4)]
+    for b in bufs:
+        hip.fill(b, 0xAB, 8 * nb)
+    rec = hip.malloc(n * mc.TRACE_DTYPE.itemsize)
+    st = hip.stream()
+    try:
+        r = eng.forward_device(*bufs, n_histories=n, stream=st, trace_ptr=rec, trace=n)
+        assert r["flights"] == host.flights
+        got = [hip.download(b, np.empty(nb)) for b in bufs]
+        # two runs add their histories' FP64 moments in different orders
+        for g, want in zip(got, (host.sum, host.sq, host.mean, host.uncer)):
+            np.testing.assert_allclose(g, want, rtol=1e-12, atol=1e-300)
+        # the device's mean and uncertainty are the host formulas on its own sums, bit for bit
+        dsum, dsq, dmean, duncer = got
+        mean = dsum / n
+        assert (dmean == mean).all()
+        assert (duncer == np.sqrt((dsq / n - mean * mean) / (n - 1.0))).all()
+        tr = hip.download(rec, np.empty(n, dtype=mc.TRACE_DTYPE))
+        same_records(tr, host.trace, "device trace")
+        # backward with device-side weights and outputs
+        go = np.random.default_rng(9).normal(size=nb)
+        hb = eng.backward(go, n_histories=n)
+        gd = hip.upload(go)
+        g, gs = hip.malloc(8 * npar), hip.malloc(8 * npar)
+        eng.backward_device(gd, g, gs, n_histories=n, stream=st)
+        np.testing.assert_allclose(hip.download(g, np.empty(npar)), hb.grad, rtol=1e-10, atol=1e-14)
+        np.testing.assert_allclose(hip.download(gs, np.empty(npar)), hb.grad_sigma, rtol=1e-8)
+        for b in (gd, g, gs):
+            hip.free(b)
+    finally:
+        hip.stream_destroy(st)
+        for b in bufs + [rec]:
+            hip.free(b)
 
 
 def test_problem_from_flat_arrays_runs_on_the_gpu():
