@@ -300,6 +300,17 @@ mc_status mc_backward(mc_engine* e, const mc_run_cfg* run, const double* grad_ou
 mc_status mc_xs_lookup(mc_engine* e, const double* energy, const int32_t* mat,
                        double* sigt, uint64_t n);
 
+/* mc_xs_lookup without any device allocation, for callers that cannot
+ * allocate (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The device staging of
+ * the inputs and outputs is a fixed array in the library, one per device, held
+ * until the process ends: n <= 524288 (2^19) lookups per call, larger n is
+ * rejected with MC_ERR_CAPACITY.  Calls that overlap on one device share that
+ * array and race; calls on different devices do not.  Same results as
+ * mc_xs_lookup, bit for bit.                                                  */
+mc_status mc_xs_lookup_noalloc(mc_engine* e, const double* energy, const int32_t* mat, double* sigt,
+                               uint64_t n);
+
 /* Times the lookup kernel alone on n device-resident inputs (E log-uniform
  * over the tabulated range, material uniform over problem 0's materials):
  * average seconds per launch over `iters` launches; checksum of the outputs.

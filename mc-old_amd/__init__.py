@@ -117,6 +117,8 @@ EXPORTS = {
                               C.POINTER(C.c_double), C.POINTER(TallyOut)]),
     "mc_xs_lookup": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                C.c_uint64]),
+    "mc_xs_lookup_noalloc": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_double), C.c_uint64]),
     "mc_xs_lookup_bench": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_uint64, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
     "mc_selftest_math": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64]),

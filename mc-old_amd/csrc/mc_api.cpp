@@ -193,6 +193,14 @@ mc_status mc_xs_lookup(mc_engine* e, const double* energy, const int32_t* mat, d
     });
 }
 
+mc_status mc_xs_lookup_noalloc(mc_engine* e, const double* energy, const int32_t* mat, double* sigt, uint64_t n)
+{
+    return guarded([&] {
+        if (!e || (n && (!energy || !mat || !sigt))) throw mc::Error(MC_ERR_ARGUMENT, "null argument");
+        if (n) e->E->xs_lookup_noalloc(energy, mat, sigt, n);
+    });
+}
+
 mc_status mc_xs_lookup_bench(mc_engine* e, uint64_t n, int32_t iters, uint64_t seed, double* seconds,
                              double* checksum)
 {

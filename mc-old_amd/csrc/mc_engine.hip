@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,34 @@ __global__ void k_selftest_math(const double* in, double* out, uint64_t n);
         if (_e != hipSuccess)                                                                 \
             throw Error(MC_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(_e));      \
     } while (0)
+
+// ------------------------------------------------- allocation-free lookups
+// mc_xs_lookup_noalloc stages its inputs and outputs in this array instead of
+// allocating device buffers: energies at byte 0, Sigma_t at 8 kScratchLookups,
+// materials at 16 kScratchLookups.  Each device holds its own copy from the
+// module's load until the process ends; every call on a device reuses that
+// device's copy, so calls that overlap on one device race.  Nothing in it
+// needs to start at zero: a call writes every entry it reads.
+constexpr uint64_t kScratchLookups = 1ull << 19;
+constexpr size_t kScratchBytes = (size_t)kScratchLookups * 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The device address of `scratch` on device dev (the array's name in host
+// code is only a stand-in), looked up once per device with dev current.
+static unsigned char* device_scratch(int dev)
+{
+    static std::mutex mu;
+    static std::map<int, unsigned char*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw Error(MC_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    addr[dev] = (unsigned char*)p;
+    return (unsigned char*)p;
+}
 
 // ----------------------------------------------------------- device buffers
 // Blocks of the per-run key-to-problem index.
@@ -86,6 +115,7 @@ static uint64_t lcg_pow(uint64_t g, uint64_t e)
 
 struct Engine::Impl {
     int device = 0;
+    unsigned char* scratch_ptr = nullptr;   // this device's `scratch` (allocation-free lookups)
     hipStream_t stream = nullptr;      // the engine's own stream (graphs are captured on it)
     hipStream_t cur = nullptr;         // the stream of the current run
     int n_cu = 0;
@@ -1231,6 +1261,7 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     if ((int64_t)d->n_cu * cfg.grid_per_cu > 8192 || cfg.grid_per_cu < 1)
         throw Error(MC_ERR_ARGUMENT, "grid_per_cu gives more than 8192 workgroups");
     HIP_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    d->scratch_ptr = device_scratch(d->device);
     HIP_CHECK(hipHostMalloc((void**)&d->h_ctr, sizeof(Counters), hipHostMallocDefault));
     HIP_CHECK(hipEventCreate(&d->ev0));
     HIP_CHECK(hipEventCreate(&d->ev1));
@@ -1294,6 +1325,30 @@ void Engine::xs_lookup(const double* E, const int32_t* mat, double* out, uint64_
                        bM.as<int32_t>(), bO.as<double>(), n);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipMemcpyAsync(out, bO.p, 8 * n, hipMemcpyDeviceToHost, d->stream));
+    HIP_CHECK(hipStreamSynchronize(d->stream));
+}
+
+// mc_xs_lookup with the device staging in `scratch` (at this device's address
+// found when the engine was created): no device memory is allocated or freed.
+void Engine::xs_lookup_noalloc(const double* E, const int32_t* mat, double* out, uint64_t n)
+{
+    if (n > kScratchLookups)
+        throw Error(MC_ERR_CAPACITY, "mc_xs_lookup_noalloc takes at most " + std::to_string(kScratchLookups) +
+                                         " lookups per call (" + std::to_string(n) + " given)");
+    HIP_CHECK(hipSetDevice(d->device));
+    for (uint64_t i = 0; i < n; ++i)
+        if (mat[i] < -1 || mat[i] >= (int32_t)d->n_mat0)
+            throw Error(MC_ERR_ARGUMENT, "material index out of range");
+    double* sE = (double*)d->scratch_ptr;
+    double* sO = (double*)(d->scratch_ptr + 8 * kScratchLookups);
+    int32_t* sM = (int32_t*)(d->scratch_ptr + 16 * kScratchLookups);
+    HIP_CHECK(hipMemcpyAsync(sE, E, 8 * n, hipMemcpyHostToDevice, d->stream));
+    HIP_CHECK(hipMemcpyAsync(sM, mat, 4 * n, hipMemcpyHostToDevice, d->stream));
+    const KArgs A = d->args(false, nullptr, 0);
+    hipLaunchKernelGGL(k_xs_lookup_array<kXsUnroll>, d->lookup_grid(), d->lookup_block(), d->lookup_lds(), d->stream, A,
+                       (const double*)sE, (const int32_t*)sM, sO, n);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(out, sO, 8 * n, hipMemcpyDeviceToHost, d->stream));
     HIP_CHECK(hipStreamSynchronize(d->stream));
 }
 

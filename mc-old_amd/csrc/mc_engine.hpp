@@ -19,6 +19,7 @@ public:
     void forward(const mc_run_cfg* run, mc_tally_out* out);
     void backward(const mc_run_cfg* run, const double* grad_out, double* grad, double* sigma, mc_tally_out* out);
     void xs_lookup(const double* E, const int32_t* mat, double* out, uint64_t n);
+    void xs_lookup_noalloc(const double* E, const int32_t* mat, double* out, uint64_t n);
     void xs_bench(uint64_t n, int iters, uint64_t seed, double* seconds, double* checksum);
     void set_cfg(const mc_engine_cfg& cfg);
     void kernel_stats(mc_kernel_stats* out, bool reset);
