@@ -90,19 +90,23 @@ def test_shield_gradient_vs_crn_finite_differences(response):
 
     A CRN difference sees the part of a derivative that acts through
     discrete decisions (a flight that crosses a surface instead of
-    colliding, a roulette survival) only in the histories whose decision the
-    step flips.  A component whose flips the sample holds (MIN_FLIPS or more)
-    is held to the 2-sigma bar.  One whose step flips none (the capture
-    scale factors of the transmitted fraction: 1/v capture is ~1e-5 of the
-    total cross section at the energies that transmit, so even a step of 0.5
-    flips nothing in 2e6 histories) gets from the differences only the
-    smooth part of its derivative: the part carried by the crossing
-    probabilities exp(-Sigma_t d), which the likelihood ratio holds as
-    -d dSigma_t/dtheta, is missing there, and the two are not compared;
-    their likelihood-ratio values are held to the oracle's to 1e-9
-    (test_shield_backward_matches_oracle) and capture scale factors to
-    differences in test_capture_gradient_vs_crn_finite_differences."""
-    n, h = 2_000_000, 1e-3
+    colliding, a nuclide choice, a roulette survival) only in the histories
+    whose decision the step flips.  A component whose flips the sample
+    holds (MIN_FLIPS or more: the scatter factors of the nuclides that make
+    up each layer) is held to the 2-sigma bar.  One whose step flips none
+    (the capture factors: 1/v capture is ~1e-5 of the total cross section at
+    the energies that transmit, and even a step of 0.5 flips nothing in
+    1e5 histories; the scatter factors of trace nuclides) gets from the
+    differences only the smooth part of its derivative: the part carried by
+    the crossing probabilities exp(-Sigma_t d), which the likelihood ratio
+    holds as -d dSigma_t/dtheta, is missing there (the differences of the
+    water capture factors are 0.43 of the likelihood ratio's on the
+    transmitted fraction), so the two are not compared; those
+    likelihood-ratio values are held to the oracle's to 1e-9
+    (test_shield_backward_matches_oracle), and capture factors to
+    differences in test_capture_gradient_vs_crn_finite_differences, where
+    capture decides many crossings."""
+    n, h = 8_000_000, 1e-3
     deck = decks.shield_deck(n, params="24", leakage=response == "leakage",
                              transmission=response == "transmission")
     p = mc.Problem.from_string(deck, shield_xs_dir())
@@ -118,9 +122,9 @@ def test_shield_gradient_vs_crn_finite_differences(response):
         print(f"{response} theta_{k:02d}: LR {b.grad[k]:+.5e} +- {b.grad_sigma[k]:.2e}  "
               f"FD {fd[k]:+.5e} +- {fd_sig[k]:.2e}  flips {flips[k]:6d}  z {z[k]:+.2f}"
               f"{'' if checked[k] else '  (no flips: not compared)'}", flush=True)
-    # every scatter scale factor changes enough decisions to be compared
-    assert checked[0::2].all(), flips
-    assert checked.sum() >= 12
+    # the scatter factors of each layer's main nuclides (water H and O, iron
+    # Fe, concrete H, C and O) change enough decisions to be compared
+    assert checked[[0, 4, 14, 16, 18, 20]].all(), flips
     assert np.all(np.abs(z[checked]) <= 2.0), z[checked]
 
 
@@ -194,7 +198,7 @@ def test_capture_gradient_vs_crn_finite_differences():
     of CRN central differences (h = 1e-3)."""
     from mc_testlib import REFERENCE_DECK
     import os
-    n, h = 2_000_000, 1e-3
+    n, h = 8_000_000, 1e-3
     p = mc.Problem.from_string(capture_slab_deck(n), os.path.join(REFERENCE_DECK, "xs_library"))
     eng = mc.Engine(p)
     P = p.n_params
@@ -206,7 +210,7 @@ def test_capture_gradient_vs_crn_finite_differences():
         print(f"capture slab theta_{k}: LR {b.grad[k]:+.5e} +- {b.grad_sigma[k]:.2e}  "
               f"FD {fd[k]:+.5e} +- {fd_sig[k]:.2e}  flips {flips[k]}  z {z[k]:+.2f}", flush=True)
     checked = flips >= MIN_FLIPS
-    assert checked[1] and checked[0::2].all(), flips      # B-10 capture and every scatter factor
+    assert checked[[0, 1, 2, 4]].all(), flips            # B-10 capture and every scatter factor
     assert abs(b.grad[1]) > 10 * b.grad_sigma[1]         # B-10 capture is resolved
     assert np.all(np.abs(z[checked]) <= 2.0), z
 
