@@ -214,8 +214,21 @@ def xs_lookup_bench(mc, decks, xs, n_xs=1 << 26):
     sec, _ = eng.xs_bench(n_xs, iters=20)
     gbs = 20 * n_xs / sec / 1e9
     log(f"xs lookup: {n_xs / sec:.4g} lookups/s, {gbs:.1f} GB/s ({gbs / HBM_PEAK_GBS:.1%})")
+    # the random-gather ceiling: the same streams and one (or two dependent)
+    # 128-byte line reads per lookup from a table of the lookup's own size
+    c1, table = eng.gather_bench(n_xs, 1)
+    c2, _ = eng.gather_bench(n_xs, 2)
+    log(f"gather ceiling: 1 line {n_xs / c1:.4g}/s, 2 dependent lines {n_xs / c2:.4g}/s, table {table / 2**20:.2f} MiB")
     return {"lookups": n_xs, "seconds_per_launch": sec, "lookups_per_s": n_xs / sec,
-            "algo_bytes_per_lookup": 20, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+            "algo_bytes_per_lookup": 20, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "gather_ceiling": {"table_bytes": table,
+                               "one_line_lookups_per_s": n_xs / c1, "one_line_GBs": 20 * n_xs / c1 / 1e9,
+                               "two_dependent_lines_lookups_per_s": n_xs / c2,
+                               "two_dependent_lines_GBs": 20 * n_xs / c2 / 1e9,
+                               "lookup_frac_of_one_line_ceiling": c1 / sec,
+                               "definition": "same E/material/output streams, 1 or 2 dependent 128-byte "
+                                             "line reads per lookup at hashed positions of a table the "
+                                             "lookup's size, no search, no arithmetic"}}
 
 
 def kernel_roofline(mc, prob, H, traffic):

@@ -286,6 +286,15 @@ mc_status mc_engine_param_offset(const mc_engine* e, int32_t p, int32_t* offset,
 /* Accumulated per-kernel statistics since the last reset (reset != 0 clears
  * them after reading).  No reference counterpart: measurement entry point. */
 mc_status mc_engine_kernel_stats(mc_engine* e, mc_kernel_stats* out, int32_t reset);
+/* The copy ring of split copies (problems whose importances rise between
+ * cells): its capacity in entries, the times it has doubled since the engine
+ * was made (the capacity is 0 until a run allocates the particle bank), and
+ * the last run's waits (splits, and later turns, that found it
+ * without room for a copy; the particle then waits at the surface, before
+ * its roulette, and places the rest of its copies as room appears).  The
+ * reference's bank grows without bound (population_control.cpp:21-49); no
+ * counterpart: measurement entry point. */
+mc_status mc_engine_copy_ring(const mc_engine* e, uint64_t* capacity, uint64_t* growths, uint64_t* waits);
 
 mc_status mc_forward(mc_engine* e, const mc_run_cfg* run, mc_tally_out* out);
 
@@ -317,6 +326,15 @@ mc_status mc_xs_lookup_noalloc(mc_engine* e, const double* energy, const int32_t
  * No reference counterpart: measurement entry point.                       */
 mc_status mc_xs_lookup_bench(mc_engine* e, uint64_t n, int32_t iters, uint64_t seed,
                              double* seconds, double* checksum);
+
+/* The random-gather ceiling of mc_xs_lookup_bench: the same input and
+ * output streams on the same n synthetic inputs, and per lookup
+ * `dependent_reads` (1 or 2) dependent reads of a 128-byte line at a hashed
+ * position in a table of table_bytes (0: the size of what the lookup reads,
+ * returned in *lookup_table_bytes), no search and no arithmetic: average
+ * seconds per launch.  No reference counterpart: measurement entry point.   */
+mc_status mc_gather_bench(mc_engine* e, uint64_t n, int32_t iters, uint64_t seed, uint64_t table_bytes,
+                          int32_t dependent_reads, double* seconds, uint64_t* lookup_table_bytes);
 
 /* Evaluates the engine's log, cos/sin(2 pi x) and sqrt on the device:
  * out[4i..4i+3] for in[i].  No reference counterpart: CPU/GPU equality test. */

@@ -112,6 +112,8 @@ EXPORTS = {
     "mc_engine_bin_offset": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "mc_engine_param_offset": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "mc_engine_kernel_stats": (C.c_int, [C.c_void_p, C.POINTER(KernelStats), C.c_int32]),
+    "mc_engine_copy_ring": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64)]),
     "mc_forward": (C.c_int, [C.c_void_p, C.POINTER(RunCfg), C.POINTER(TallyOut)]),
     "mc_backward": (C.c_int, [C.c_void_p, C.POINTER(RunCfg), C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_double), C.POINTER(TallyOut)]),
@@ -121,6 +123,8 @@ EXPORTS = {
                                        C.POINTER(C.c_double), C.c_uint64]),
     "mc_xs_lookup_bench": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_uint64, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
+    "mc_gather_bench": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_uint64, C.c_uint64, C.c_int32,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "mc_selftest_math": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64]),
     "mc_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(TuneResult)]),
     "mc_engine_destroy": (None, [C.c_void_p]),
@@ -360,6 +364,14 @@ class Engine:
         return {n: dict(launches=k.launches[i], items=k.items[i], seconds=k.seconds[i])
                 for i, n in enumerate(KERNEL_NAMES)}
 
+    def copy_ring(self) -> dict:
+        """The split-copy ring: capacity (entries), growths (doublings since
+        the engine was made) and waits (the last run's splits and turns that
+        found it without room)."""
+        cap, gr, wt = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib.mc_engine_copy_ring(self._h, C.byref(cap), C.byref(gr), C.byref(wt)))
+        return {"capacity": cap.value, "growths": gr.value, "waits": wt.value}
+
     def xs_lookup(self, energy, mat) -> np.ndarray:
         E = np.ascontiguousarray(energy, dtype=np.float64)
         M = np.ascontiguousarray(mat, dtype=np.int32)
@@ -371,6 +383,14 @@ class Engine:
         sec, chk = C.c_double(), C.c_double()
         _check(lib.mc_xs_lookup_bench(self._h, n, iters, seed, C.byref(sec), C.byref(chk)))
         return sec.value, chk.value
+
+    def gather_bench(self, n: int, dependent_reads: int = 1, table_bytes: int = 0, iters: int = 20,
+                     seed: int = 12345) -> tuple[float, int]:
+        """(seconds per launch, lookup table bytes) of the random-gather
+        ceiling kernel (mc_gather_bench)."""
+        sec, tb = C.c_double(), C.c_uint64()
+        _check(lib.mc_gather_bench(self._h, n, iters, seed, table_bytes, dependent_reads, C.byref(sec), C.byref(tb)))
+        return sec.value, tb.value
 
     def autotune(self, sample_histories: int) -> TuneResult:
         r = TuneResult()

@@ -168,6 +168,14 @@ mc_status mc_engine_kernel_stats(mc_engine* e, mc_kernel_stats* out, int32_t res
     });
 }
 
+mc_status mc_engine_copy_ring(const mc_engine* e, uint64_t* capacity, uint64_t* growths, uint64_t* waits)
+{
+    return guarded([&] {
+        if (!e) throw mc::Error(MC_ERR_ARGUMENT, "null engine");
+        e->E->copy_ring(capacity, growths, waits);
+    });
+}
+
 mc_status mc_forward(mc_engine* e, const mc_run_cfg* run, mc_tally_out* out)
 {
     return guarded([&] {
@@ -207,6 +215,17 @@ mc_status mc_xs_lookup_bench(mc_engine* e, uint64_t n, int32_t iters, uint64_t s
     return guarded([&] {
         if (!e || !seconds || n == 0) throw mc::Error(MC_ERR_ARGUMENT, "bad argument");
         e->E->xs_bench(n, iters, seed, seconds, checksum);
+    });
+}
+
+mc_status mc_gather_bench(mc_engine* e, uint64_t n, int32_t iters, uint64_t seed, uint64_t table_bytes,
+                          int32_t dependent_reads, double* seconds, uint64_t* lookup_table_bytes)
+{
+    return guarded([&] {
+        if (!e || !seconds || n == 0) throw mc::Error(MC_ERR_ARGUMENT, "bad argument");
+        if (lookup_table_bytes) *lookup_table_bytes = e->E->lookup_table_bytes();
+        const uint64_t tb = table_bytes ? table_bytes : e->E->lookup_table_bytes();
+        e->E->gather_bench(n, iters, seed, tb, dependent_reads, seconds);
     });
 }
 

@@ -27,7 +27,9 @@ constexpr int kBucketShift = 52 - kBucketBits;
 // ST_HELD: a member of a split family ended here; the slot has no particle,
 // but its tally row is a family's row while fcount[slot] > 0 (then only split
 // copies, which score into their own family's row, may take the slot).
-enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4, ST_HELD = 5 };
+// ST_WAIT: the particle split at a surface and waits there for room in the
+// copy ring for the rest of its copies (k_resume retries each iteration).
+enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4, ST_HELD = 5, ST_WAIT = 6 };
 
 constexpr int kTouched = 8;        // per-slot list of tally bins touched by the history
 
@@ -75,8 +77,9 @@ struct Counters {
     uint32_t qa_ident, pad;       // 1: every slot below active_cap is live and queue A is
                                   // the identity (not written; kernels use the slot index)
     uint64_t chead, ctail;        // copy ring: split copies waiting for a slot are [chead, ctail)
-    uint32_t cplaced, err_ring;   // copies the last refill placed; ring overflows
+    uint32_t cplaced, pad2;       // copies the last refill placed
     uint64_t xfl;                 // flights taken beyond the first of an iteration (included in tA)
+    uint64_t waits;               // splits and turns that found the ring without room
 };
 
 // Entry i of queue A.
@@ -218,6 +221,9 @@ struct KArgs {
     // the slot whose tally row the family shares (-1: none), and per row slot the
     // family's members not yet ended (copies in the ring included)
     int32_t *fam, *fcount;
+    // a particle whose split copies did not all fit in the ring waits at the
+    // surface: ppend copies still to place, the next being copy pfirst
+    int32_t *ppend, *pfirst;
     uint32_t* wg_held;            // per workgroup: ended slots whose row a family holds
     // copy ring (capacity ccap): particles made by importance splitting
     double *cx, *cy, *cz, *cu, *cv, *cw, *cE, *cwgt;

@@ -23,6 +23,7 @@ namespace mc {
 // kernels (mc_kernels.hip)
 __global__ void k_refill_count(KArgs A);
 template <bool kGrad> __global__ void k_refill_scatter(KArgs A);
+template <bool kGrad> __global__ void k_resume(KArgs A);
 __global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* c);
 __global__ void k_xs_lookup(KArgs A, int use_lds);
 template <bool kTables, bool kGrad, bool kFuse> __global__ void k_flight(KArgs A);
@@ -32,6 +33,9 @@ template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t ma
 __global__ void k_drain_close(Counters* c);
 template <int kU> __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
 constexpr int kXsUnroll = 4;      // lookups in flight per thread in the standalone kernel (measured best of 1, 2, 4)
+template <int kDep>
+__global__ void k_gather_lines(const double* E, const int32_t* mat, double* out, uint64_t n, const double* table,
+                               uint64_t nlines);
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,
                             const int32_t* mats, int32_t nmat, uint64_t seed);
 __global__ void k_fill_f64(double* p, uint64_t n, double v);
@@ -169,9 +173,11 @@ struct Engine::Impl {
         b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, b_vmat, q_S, q_A, q_C, d_ctr, d_wgcnt, b_mat,
         d_wgxfl;
     // split families and the copy ring (problems that can split only)
-    DevBuf b_fam, b_fcount, d_wgheld, r_S, r_vmat, b_famG, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
+    DevBuf b_fam, b_fcount, b_ppend, b_pfirst, d_wgheld, r_S, r_vmat, b_famG, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
         r_nfl;
     uint64_t ring_cap = 0;
+    uint64_t ring_growths = 0;         // times a run doubled the copy ring
+    uint64_t last_waits = 0;           // the last run's waits for ring room
     uint32_t h_active_cap = 0;
     uint64_t bank_cap = 0;
     bool bank_grad = false;
@@ -207,7 +213,8 @@ struct Engine::Impl {
     void ensure_partials();
     KArgs args(bool grad, mc_trace_record* dtrace, uint64_t trace_count) const;
     void launch_iteration(const KArgs& A, bool grad, hipStream_t st, hipEvent_t* ev);
-    void run_loop(bool grad, const KArgs& A);
+    void run_loop(bool grad, KArgs& A);
+    bool grow_ring(uint64_t head, uint64_t tail);
     void run(const mc_run_cfg* run, bool grad, const double* gout, mc_tally_out* out, double* grad_mean,
              double* grad_sigma);
     dim3 grid_dim() const { return dim3((unsigned)std::max(1, n_cu * cfg.grid_per_cu)); }
@@ -593,6 +600,57 @@ void Engine::Impl::upload_static()
 
 constexpr uint64_t kRingPerSlot = 8;
 
+// Copy-ring growth: entry k of a ring of capacity R sits at k mod R.
+template <class T>
+__global__ void k_ring_move(const T* __restrict__ src, T* __restrict__ dst, uint64_t head, uint64_t tail,
+                            uint64_t cap0, uint64_t cap1)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = head + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < tail; k += stride)
+        dst[k % cap1] = src[k % cap0];
+}
+
+// Doubles the copy ring, moving the copies [head, tail) to their places in
+// the larger one; false (nothing changed) when the device has no memory for
+// it or the ring holds 2^31 entries.
+bool Engine::Impl::grow_ring(uint64_t head, uint64_t tail)
+{
+    const uint64_t R0 = ring_cap, R = 2 * R0;
+    if (R > (1ull << 31)) return false;
+    const uint64_t per = 8 * 10 + 4 * 4 + (bank_grad ? 8 * (uint64_t)par_stride + 4 : 0);
+    size_t avail = 0, total = 0;
+    HIP_CHECK(hipMemGetInfo(&avail, &total));
+    if ((double)R * per > 0.8 * (double)avail) return false;   // old and new rings coexist while moving
+    const dim3 g(1024), b(256);
+    auto move = [&](DevBuf& buf, size_t esz, uint64_t rows) {
+        DevBuf nb;
+        nb.alloc(esz * R * rows);
+        for (uint64_t k = 0; k < rows; ++k) {
+            if (esz == 8)
+                hipLaunchKernelGGL(k_ring_move<uint64_t>, g, b, 0, cur, buf.as<uint64_t>() + k * R0,
+                                   nb.as<uint64_t>() + k * R, head, tail, R0, R);
+            else
+                hipLaunchKernelGGL(k_ring_move<uint32_t>, g, b, 0, cur, buf.as<uint32_t>() + k * R0,
+                                   nb.as<uint32_t>() + k * R, head, tail, R0, R);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(cur));       // before the old buffer is freed
+        std::swap(buf.p, nb.p);
+        std::swap(buf.bytes, nb.bytes);
+    };
+    for (DevBuf* r : {&r_x, &r_y, &r_z, &r_u, &r_v, &r_w, &r_E, &r_wgt, &r_rng, &r_hid}) move(*r, 8, 1);
+    for (DevBuf* r : {&r_cell, &r_prob, &r_fam, &r_nfl}) move(*r, 4, 1);
+    if (bank_grad) {
+        // rows of S are ring-major (stride = the capacity)
+        move(r_S, 8, (uint64_t)par_stride);
+        move(r_vmat, 4, 1);
+    }
+    ring_cap = R;
+    ++ring_growths;
+    ++generation;                                   // graphs hold the ring's addresses
+    return true;
+}
+
 // Row stride (elements) of the parameter-major S and H arrays for a bank of
 // cap slots: cap plus 32 KiB + 256 B, so that the rows of one slot do not sit a
 // power of two apart (a fold updates up to eight of them at once).
@@ -624,6 +682,8 @@ void Engine::Impl::ensure_bank(bool grad)
             const uint64_t R = kRingPerSlot * B;
             b_fam.alloc(4 * B);
             b_fcount.alloc(4 * B);
+            b_ppend.alloc(4 * B);
+            b_pfirst.alloc(4 * B);
             for (DevBuf* r : {&r_x, &r_y, &r_z, &r_u, &r_v, &r_w, &r_E, &r_wgt, &r_rng, &r_hid}) r->alloc(8 * R);
             for (DevBuf* r : {&r_cell, &r_prob, &r_fam, &r_nfl}) r->alloc(4 * R);
             ring_cap = R;
@@ -720,6 +780,8 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     if (split_any) {
         A.fam = b_fam.as<int32_t>();
         A.fcount = b_fcount.as<int32_t>();
+        A.ppend = b_ppend.as<int32_t>();
+        A.pfirst = b_pfirst.as<int32_t>();
         A.wg_held = d_wgheld.as<uint32_t>();
         A.cx = r_x.as<double>(); A.cy = r_y.as<double>(); A.cz = r_z.as<double>();
         A.cu = r_u.as<double>(); A.cv = r_v.as<double>(); A.cw = r_w.as<double>();
@@ -802,6 +864,10 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
         if (ev) HIP_CHECK(hipEventRecord(ev[2 * k + edge], st));
     };
     mark(0, 0);
+    if (split_any) {
+        if (grad) hipLaunchKernelGGL(k_resume<true>, g, b, 0, st, A);
+        else hipLaunchKernelGGL(k_resume<false>, g, b, 0, st, A);
+    }
     hipLaunchKernelGGL(k_refill_count, g, b, 0, st, A);
     mark(0, 1);
     mark(1, 0);
@@ -840,13 +906,13 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     if (ev) hipLaunchKernelGGL(k_count_events, dim3(1), dim3(256), 0, st, A.wg_cnt, g.x, A.ctr);
 }
 
-void Engine::Impl::run_loop(bool grad, const KArgs& A)
+void Engine::Impl::run_loop(bool grad, KArgs& A)
 {
     const int gi = grad ? 1 : 0;
     const int K = std::max(1, cfg.iters_per_graph);
     const bool profile = cfg.profile != 0;
     const bool use_graph = cfg.use_graph && !profile;
-    if (use_graph) {
+    auto ensure_graph = [&]() {
         if (!graph[gi] || graph_gen[gi] != generation || graph_iters[gi] != K ||
             graph_trace[gi] != (const void*)A.trace || graph_tcount[gi] != A.trace_count) {
             if (graph[gi]) { HIP_CHECK(hipGraphExecDestroy(graph[gi])); graph[gi] = nullptr; }
@@ -861,7 +927,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             graph_trace[gi] = (const void*)A.trace;
             graph_tcount[gi] = A.trace_count;
         }
-    }
+    };
     if (profile && kev.empty()) {
         kev.resize(2 * MC_N_KERNELS);
         for (auto& e : kev) HIP_CHECK(hipEventCreate(&e));
@@ -871,6 +937,10 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
     const dim3 g = grid_dim(), b((unsigned)cfg.block);
     uint64_t drain_flights = 0, drain_coll = 0;
     auto refill = [&]() {
+        if (split_any) {
+            if (grad) hipLaunchKernelGGL(k_resume<true>, g, b, 0, cur, A);
+            else hipLaunchKernelGGL(k_resume<false>, g, b, 0, cur, A);
+        }
         hipLaunchKernelGGL(k_refill_count, g, b, 0, cur, A);
         if (grad) hipLaunchKernelGGL(k_refill_scatter<true>, g, b, 0, cur, A);
         else hipLaunchKernelGGL(k_refill_scatter<false>, g, b, 0, cur, A);
@@ -879,6 +949,30 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(h_ctr, A.ctr, sizeof(Counters), hipMemcpyDeviceToHost, cur));
         HIP_CHECK(hipStreamSynchronize(cur));
+    };
+    // Back-pressure of the copy ring: a split whose copies do not all fit
+    // leaves its particle waiting at the surface until they do.  Once a chunk
+    // has seen a particle wait, the ring doubles (its copies move to their
+    // places in the larger ring).  A refill that queued no particle while the
+    // ring holds copies found every slot waiting: the ring must grow, and a
+    // device without the memory for it ends the run.
+    uint64_t waits_seen = 0;                     // the run's counters start at zero
+    // returns true when the run is over
+    auto settle = [&]() -> bool {
+        const uint64_t held = split_any ? h_ctr->ctail - (h_ctr->chead + h_ctr->cplaced) : 0;
+        const bool stuck = h_ctr->nA == 0 && held > 0;
+        if (h_ctr->nA == 0 && !stuck) return true;
+        if (split_any && (stuck || h_ctr->waits != waits_seen)) {
+            waits_seen = h_ctr->waits;
+            if (grow_ring(h_ctr->chead + h_ctr->cplaced, h_ctr->ctail)) {
+                A = args(grad, A.trace, A.trace_count);
+            } else if (stuck) {
+                throw Error(MC_ERR_CAPACITY, "the copy ring (" + std::to_string(ring_cap) +
+                                                 " entries) is full, every particle waits for room in it, "
+                                                 "and the device has no memory to grow it");
+            }
+        }
+        return false;
     };
     for (uint64_t chunk = 0;; ++chunk) {
         // The tail of a run: every history has started and few particles are
@@ -915,13 +1009,14 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
                 kstats.seconds[kDrainSlot] += 1e-3 * ms;
                 kstats.launches[kDrainSlot] += 1;
             }
-            if (h_ctr->nA == 0) break;
+            if (settle()) break;
             // histories longer than the drain's event limit: back to iterations,
             // whose first roll-over must not count this refill's particles
             hipLaunchKernelGGL(k_drain_close, dim3(1), dim3(64), 0, cur, A.ctr);
             continue;
         }
         if (use_graph) {
+            ensure_graph();
             HIP_CHECK(hipGraphLaunch(graph[gi], cur));
         } else if (profile) {
             for (int k = 0; k < K; ++k) {
@@ -943,7 +1038,7 @@ void Engine::Impl::run_loop(bool grad, const KArgs& A)
             for (int k = 0; k < K; ++k) launch_iteration(A, grad, cur, nullptr);
         }
         read_ctr();
-        if (h_ctr->nA == 0) break;
+        if (settle()) break;
         const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (el > max_s)
             throw Error(MC_ERR_CAPACITY, "event loop exceeded " + std::to_string(max_s) +
@@ -1106,7 +1201,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         hipLaunchKernelGGL(k_fill_u8, dim3(1024), dim3(256), 0, cur, (uint8_t*)dtrace, (uint64_t)bytes, (uint8_t)0);
     }
     ensure_partials();
-    const KArgs A = args(grad, dtrace, tcount);
+    KArgs A = args(grad, dtrace, tcount);     // run_loop renews it when the copy ring grows
     const dim3 fg(1024), fb(256);
     const uint64_t nbp = part_rows * (uint64_t)std::max(1, n_bins), npp = part_rows * (uint64_t)std::max(1, n_params);
     hipLaunchKernelGGL(k_fill_f64, fg, fb, 0, cur, A.psum, nbp, 0.0);
@@ -1126,7 +1221,8 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     HIP_CHECK(hipEventRecord(ev0, cur));
     const bool plain = run && run->mode == MC_RUN_PLAIN;
     uint64_t flights = 0;
-    uint32_t err_lost = 0, err_split = 0, err_ring = 0;
+    uint32_t err_lost = 0, err_split = 0;
+    uint64_t waits = 0;
     for (int p = 0; p < (plain ? np : 1); ++p) {
         Counters c{};
         c.next_hist = plain ? hb[p] : 0;
@@ -1140,23 +1236,22 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         if (split_any) {
             hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.fam, bank_cap, -1);
             hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.fcount, bank_cap, 0);
+            hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.ppend, bank_cap, 0);
         }
         run_loop(grad, A);
         flights += h_ctr->tA;      // every entry of queue A is one flight
+        waits += h_ctr->waits;
         err_lost += h_ctr->err_lost;
         err_split += h_ctr->err_split;
-        err_ring += h_ctr->err_ring;
     }
     HIP_CHECK(hipEventRecord(ev1, cur));
     HIP_CHECK(hipEventSynchronize(ev1));
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    last_waits = waits;
     if (err_lost) throw Error(MC_ERR_LOST_PARTICLE, std::to_string(err_lost) + " particle(s) lost");
     if (err_split)
         throw Error(MC_ERR_ARGUMENT, "a particle split in a problem without an importance rise");
-    if (err_ring)
-        throw Error(MC_ERR_CAPACITY, "split copies overflowed the copy ring (" + std::to_string(ring_cap) +
-                                         " entries, 8 per bank slot); raise bank_capacity");
     // reduce the per-workgroup partial moments (workgroups in order)
     const dim3 rg(256), rb(256);
     hipLaunchKernelGGL(k_reduce_rows, rg, rb, 0, cur, A.psum, part_rows, (uint64_t)n_bins, t_sum.as<double>());
@@ -1389,6 +1484,58 @@ void Engine::xs_bench(uint64_t n, int iters, uint64_t seed, double* sec, double*
         for (double v : o) s += v;
         *checksum = s;
     }
+}
+
+// Times k_gather_lines on the same synthetic inputs as xs_bench: `dep`
+// dependent 128-byte line reads per lookup from a table of table_bytes.
+void Engine::gather_bench(uint64_t n, int iters, uint64_t seed, uint64_t table_bytes, int dep, double* sec)
+{
+    HIP_CHECK(hipSetDevice(d->device));
+    if (dep != 1 && dep != 2) throw Error(MC_ERR_ARGUMENT, "dependent reads per lookup must be 1 or 2");
+    const uint64_t nlines = std::max<uint64_t>(1, table_bytes / 128);
+    std::vector<int32_t> mats{0};
+    DevBuf bE, bM, bO, bT, bmats;
+    bE.alloc(8 * n);
+    bM.alloc(4 * n);
+    bO.alloc(8 * n);
+    bT.alloc(128 * nlines);
+    upload(bmats, mats, d->stream);
+    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, d->stream, bT.as<double>(), 16 * nlines, 1.0);
+    const double lnmin = std::log(d->emin > 0 ? d->emin : 1e-5), lnspan = std::log(std::max(d->emax, 2.0)) - lnmin;
+    hipLaunchKernelGGL(k_xs_inputs, dim3(2048), dim3(256), 0, d->stream, bE.as<double>(), bM.as<int32_t>(), n,
+                       lnmin, lnspan, bmats.as<int32_t>(), 1, seed);
+    const dim3 g = d->lookup_grid(), b = d->lookup_block();
+    auto launch = [&]() {
+        if (dep == 1)
+            hipLaunchKernelGGL(k_gather_lines<1>, g, b, 0, d->stream, bE.as<double>(), bM.as<int32_t>(),
+                               bO.as<double>(), n, bT.as<double>(), nlines);
+        else
+            hipLaunchKernelGGL(k_gather_lines<2>, g, b, 0, d->stream, bE.as<double>(), bM.as<int32_t>(),
+                               bO.as<double>(), n, bT.as<double>(), nlines);
+    };
+    launch();
+    HIP_CHECK(hipEventRecord(d->ev0, d->stream));
+    for (int k = 0; k < iters; ++k) launch();
+    HIP_CHECK(hipEventRecord(d->ev1, d->stream));
+    HIP_CHECK(hipEventSynchronize(d->ev1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    *sec = ms * 1e-3 / std::max(1, iters);
+}
+
+uint64_t Engine::lookup_table_bytes() const
+{
+    // what the standalone lookup reads besides its streams: the records of
+    // the LDS path (or the nuclide tables) and the grids
+    return 8 * (uint64_t)(d->rec_te.size() + d->rec_t.size() + d->grid_E.size());
+}
+
+void Engine::copy_ring(uint64_t* capacity, uint64_t* growths, uint64_t* waits) const
+{
+    if (capacity) *capacity = d->ring_cap;
+    if (growths) *growths = d->ring_growths;
+    if (waits) *waits = d->last_waits;
 }
 
 void Engine::kernel_stats(mc_kernel_stats* out, bool reset)

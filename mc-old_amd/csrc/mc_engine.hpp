@@ -21,8 +21,11 @@ public:
     void xs_lookup(const double* E, const int32_t* mat, double* out, uint64_t n);
     void xs_lookup_noalloc(const double* E, const int32_t* mat, double* out, uint64_t n);
     void xs_bench(uint64_t n, int iters, uint64_t seed, double* seconds, double* checksum);
+    void gather_bench(uint64_t n, int iters, uint64_t seed, uint64_t table_bytes, int dep, double* seconds);
+    uint64_t lookup_table_bytes() const;
     void set_cfg(const mc_engine_cfg& cfg);
     void kernel_stats(mc_kernel_stats* out, bool reset);
+    void copy_ring(uint64_t* capacity, uint64_t* growths, uint64_t* waits) const;
     mc_engine_cfg cfg() const;
     int n_problems() const;
     uint64_t deck_histories(int p) const;

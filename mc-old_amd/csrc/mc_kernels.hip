@@ -1,6 +1,9 @@
 // mc_kernels.hip — event-based transport kernels for gfx950 (wave64).
 //
 // One event iteration over the structure-of-arrays particle bank:
+//   k_resume          (problems that can split) particles waiting at a surface
+//                     for room in the copy ring place the copies it now has
+//                     room for; those with none left take their roulette
 //   k_refill_count    per-workgroup counts of live, free and family-held slots
 //                     (and the counter roll-over from the previous iteration)
 //   k_refill_scatter  prefix over the counts; split copies waiting in the
@@ -747,7 +750,7 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
     // split copies waiting in the ring go first: a refill that has any places
     // copies only, into free slots and slots whose row a family holds
     const uint64_t chead = A.ctr->chead;
-    const uint64_t pend = A.split_any && A.ctr->err_ring == 0 ? A.ctr->ctail - chead : 0;
+    const uint64_t pend = A.split_any ? A.ctr->ctail - chead : 0;
     const bool cmode = pend > 0;
     uint32_t live_before, free_before, live_tot, free_tot;
     wg_prefix(A.wg_cnt, live_before, free_before, live_tot, free_tot, lds);
@@ -942,30 +945,52 @@ __device__ __forceinline__ void weight_roulette(double& w, bool& alive, uint64_t
 // wait in the copy ring for a slot; copy i (from 0) starts (i + 1) 2^50 draws
 // ahead of the particle's stream.  The family's members score into the tally
 // row of the slot where the family began, and the last member to end closes
-// it out.  Gradient runs do not split (the count reports it).
-template <bool kGrad>
-__device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, Vec3 p, Vec3 u, double E, double w,
-                                          uint64_t rng, uint32_t nfl, int cell, int p_id)
+// it out.
+//
+// Back-pressure: copies the ring has no room for stay with the particle,
+// which waits at the surface (ppend[s] copies still to place, the next being
+// copy pfirst[s]) and places them at its next turns, as the refill moves
+// copies into free slots; it flies on once all are placed.  Its state is the
+// state at the split until then, so every copy is the one an immediate
+// placement would have made.  The host grows the ring when particles wait.
+
+// The particle of slot s becomes a member of a split family of n members.
+__device__ __forceinline__ void join_family(const KArgs& A, uint32_t s, int n)
 {
-    if (!A.split_any) {                      // (cannot happen: a split needs an importance rise)
-        atomicAdd(&A.ctr->err_split, 1u);
-        return;
-    }
-    const uint32_t nc = (uint32_t)(n - 1);
-    const uint64_t t0 = atomicAdd((unsigned long long*)&A.ctr->ctail, (unsigned long long)nc);
-    if (t0 + nc - A.ctr->chead > A.ccap) {
-        // no room: the run reports the overflow and places no further copies
-        atomicAdd(&A.ctr->err_ring, 1u);
-        return;
-    }
-    int f = A.fam[s];
+    const int f = A.fam[s];
     if (f < 0) {
-        f = (int)s;
-        A.fam[s] = f;
-        atomicAdd(&A.fcount[f], n);          // the particle and its copies
+        A.fam[s] = (int)s;
+        atomicAdd(&A.fcount[s], n);          // the particle and its copies
     } else {
-        atomicAdd(&A.fcount[f], (int)nc);
+        atomicAdd(&A.fcount[f], n - 1);
     }
+}
+
+// Places copies first .. first + count - 1 of the split of slot s's particle
+// (state p, u, E, w, rng, nfl in cell `cell`) in the ring, as many as there is
+// room for, and returns how many.  Only the refill takes copies out of the
+// ring, so outside it the room can only shrink: a compare-and-swap on the
+// tail reserves a run of entries.
+template <bool kGrad>
+__device__ uint32_t push_copies(const KArgs& A, uint32_t s, uint32_t first, uint32_t count, Vec3 p, Vec3 u,
+                                double E, double w, uint64_t rng, uint32_t nfl, int cell, int p_id)
+{
+    // copies the last refill placed have left the ring (the head is rolled
+    // past them at the next refill)
+    const uint64_t head = A.ctr->chead + A.ctr->cplaced;
+    unsigned long long* tailp = (unsigned long long*)&A.ctr->ctail;
+    unsigned long long t0 = __hip_atomic_load(tailp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t take = 0;
+    for (;;) {
+        const uint64_t used = (uint64_t)t0 - head;
+        const uint64_t room = used < A.ccap ? A.ccap - used : 0;
+        take = room < count ? (uint32_t)room : count;
+        if (take == 0) return 0;
+        const unsigned long long prev = atomicCAS(tailp, t0, t0 + take);
+        if (prev == t0) break;
+        t0 = prev;
+    }
+    const int f = A.fam[s];
     const uint64_t hid = A.hid[s];
     if (kGrad) {
         // the copies carry the particle's score function S (its H, Q and
@@ -975,15 +1000,16 @@ __device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, 
         uint32_t vm = A.vmat[s];
         const int m1 = A.cell[cell].material;
         if (m1 >= 0 && A.mat[m1].param_count > 0) vm |= mat_bit(PR, m1);
-        for (uint32_t i = 0; i < nc; ++i) A.rvmat[(t0 + i) % A.ccap] = vm;
+        for (uint32_t i = 0; i < take; ++i) A.rvmat[(t0 + i) % A.ccap] = vm;
         for (int k = 0; k < PR.param_count; ++k) {
             const double v = lr_load(A.S + (size_t)k * A.sstride + s);
-            for (uint32_t i = 0; i < nc; ++i) A.rS[(size_t)k * A.ccap + (t0 + i) % A.ccap] = v;
+            for (uint32_t i = 0; i < take; ++i) A.rS[(size_t)k * A.ccap + (t0 + i) % A.ccap] = v;
         }
     }
     uint64_t r = rng;
-    for (uint32_t i = 0; i < nc; ++i) {
-        r = (r * A.jump_copy) & kRnMask;
+    for (uint32_t i = 0; i <= first; ++i) r = (r * A.jump_copy) & kRnMask;    // copy `first`
+    for (uint32_t i = 0; i < take; ++i) {
+        if (i > 0) r = (r * A.jump_copy) & kRnMask;
         const uint64_t k = (t0 + i) % A.ccap;
         A.cx[k] = p.x; A.cy[k] = p.y; A.cz[k] = p.z;
         A.cu[k] = u.x; A.cv[k] = u.y; A.cw[k] = u.z;
@@ -996,6 +1022,22 @@ __device__ __forceinline__ void spawn_copies(const KArgs& A, uint32_t s, int n, 
         A.cfam[k] = f;
         A.cnfl[k] = nfl;
     }
+    return take;
+}
+
+// A split into n at a crossing: the family grows by n - 1 and the copies go
+// to the ring; returns how many are left to place (0: all placed).
+template <bool kGrad>
+__device__ __forceinline__ uint32_t spawn_copies(const KArgs& A, uint32_t s, int n, Vec3 p, Vec3 u, double E, double w,
+                                                 uint64_t rng, uint32_t nfl, int cell, int p_id)
+{
+    if (!A.split_any) {                      // (cannot happen: a split needs an importance rise)
+        atomicAdd(&A.ctr->err_split, 1u);
+        return 0;
+    }
+    join_family(A, s, n);
+    const uint32_t nc = (uint32_t)(n - 1);
+    return nc - push_copies<kGrad>(A, s, 0, nc, p, u, E, w, rng, nfl, cell, p_id);
 }
 
 template <bool kGrad>
@@ -1005,7 +1047,7 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
     const mc_surface_desc& Sd = A.surf[sf];
     const DProb PR = A.prob[p_id];
     int cell = cell_old;
-    bool alive = true;
+    bool alive = true, waiting = false;
     if (kGrad) {
         // the flight that ended here, in the cell it leaves
         const int m = A.cell[cell_old].material;
@@ -1052,10 +1094,20 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
         } else {
             const int nsplit = (int)floor(rat + urand(rng));
             w = w / (double)nsplit;
-            if (nsplit > 1) spawn_copies<kGrad>(A, s, nsplit, p, u, E, w, rng, nfl, cell, p_id);
+            if (nsplit > 1) {
+                const uint32_t left = spawn_copies<kGrad>(A, s, nsplit, p, u, E, w, rng, nfl, cell, p_id);
+                if (left > 0) {
+                    // the particle waits here, before its roulette, with the
+                    // state its remaining copies are made from
+                    A.ppend[s] = (int32_t)left;
+                    A.pfirst[s] = nsplit - 1 - (int32_t)left;
+                    atomicAdd((unsigned long long*)&A.ctr->waits, 1ull);
+                    waiting = true;
+                }
+            }
         }
     }
-    weight_roulette(w, alive, rng, PR);
+    if (!waiting) weight_roulette(w, alive, rng, PR);
     A.x[s] = p.x; A.y[s] = p.y; A.z[s] = p.z;
     A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
     A.wgt[s] = w;
@@ -1068,9 +1120,58 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
         const uint32_t vm = A.vmat[s], bit = mat_bit(PR, m1);
         if (!(vm & bit)) A.vmat[s] = vm | bit;
     }
+    if (waiting) return ST_WAIT;
     if (alive) return ST_ALIVE;
     return finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, nfl);
 }
+
+// A turn of a particle waiting at a surface (status ST_WAIT): it places the
+// copies the ring has room for and, once all are placed, takes the roulette
+// it stopped before (surface_crossing).  Returns ST_WAIT while copies are
+// left, else ST_ALIVE (it flies on) or the status its end gives the slot.
+template <bool kGrad>
+__device__ __forceinline__ uint8_t resume_parked(const KArgs& A, uint32_t s)
+{
+    const uint32_t left = (uint32_t)A.ppend[s], first = (uint32_t)A.pfirst[s];
+    const Vec3 p = {A.x[s], A.y[s], A.z[s]};
+    const Vec3 u = {A.u[s], A.v[s], A.w[s]};
+    const double E = A.E[s];
+    double w = A.wgt[s];
+    uint64_t rng = A.rng[s];
+    const uint32_t nfl = A.nflight[s];
+    const int cell = A.pcell[s], p_id = A.pprob[s];
+    const uint32_t put = push_copies<kGrad>(A, s, first, left, p, u, E, w, rng, nfl, cell, p_id);
+    if (put < left) {
+        A.ppend[s] = (int32_t)(left - put);
+        A.pfirst[s] = (int32_t)(first + put);
+        atomicAdd((unsigned long long*)&A.ctr->waits, 1ull);
+        return ST_WAIT;
+    }
+    A.ppend[s] = 0;
+    const DProb PR = A.prob[p_id];
+    bool alive = true;
+    weight_roulette(w, alive, rng, PR);
+    A.wgt[s] = w;
+    A.rng[s] = rng;
+    if (alive) return ST_ALIVE;
+    return finish_history<kGrad, true>(A, s, p_id, PR, rng, p, E, nfl);
+}
+
+// First kernel of an iteration of a run that can split: waiting particles
+// retry before the refill counts the slots.
+template <bool kGrad>
+__global__ void __launch_bounds__(1024) k_resume(KArgs A)
+{
+    uint32_t lo, hi;
+    block_range(A.ctr->active_cap, lo, hi);
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        if (A.status[i] != ST_WAIT) continue;
+        const uint8_t st = resume_parked<kGrad>(A, i);
+        if (st != ST_WAIT) A.status[i] = st;
+    }
+}
+template __global__ void k_resume<false>(KArgs);
+template __global__ void k_resume<true>(KArgs);
 
 // One flight of slot s: distance to collision and to the nearest surface,
 // the move and its track-length scores; a flight ending on a surface crosses
@@ -1700,6 +1801,53 @@ __global__ void __launch_bounds__(1024) k_xs_lookup_array(KArgs A, const double*
     }
 }
 template __global__ void k_xs_lookup_array<4>(KArgs, const double*, const int32_t*, double*, uint64_t);
+
+// The random-gather ceiling of the standalone lookup: the same input and
+// output streams (E 8 B, material 4 B in; 8 B out, non-temporal) and, per
+// lookup, kDep dependent reads of a 128-byte line of a table of nlines lines
+// at a position hashed from E (the lookup's record reads, with no
+// arithmetic and no search).  Its rate bounds what any lookup reading that
+// many lines of a table that size can reach.
+template <int kDep>
+__global__ void __launch_bounds__(1024) k_gather_lines(const double* __restrict__ E, const int32_t* __restrict__ mat,
+                                                       double* __restrict__ out, uint64_t n,
+                                                       const double* __restrict__ table, uint64_t nlines)
+{
+    constexpr int kU = 4;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kU * stride) {
+        uint64_t h[kU];
+        int32_t m[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t iu = i + u * stride;
+            const double e = iu < n ? __builtin_nontemporal_load(E + iu) : 1.0;
+            m[u] = iu < n ? __builtin_nontemporal_load(mat + iu) : 0;
+            h[u] = double_to_bits(e) * 0x9E3779B97F4A7C15ull;
+        }
+        double r[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            uint64_t line = (h[u] >> 20) % nlines;
+            double v = 0.0;
+#pragma unroll
+            for (int d = 0; d < kDep; ++d) {
+                const double* l = table + 16 * line;
+                const double a = l[0], b = l[15];
+                v += a + b;
+                line = (line + (uint64_t)double_to_bits(a) + 1) % nlines;   // the next read depends on this one
+            }
+            r[u] = v * (double)m[u];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint64_t iu = i + u * stride;
+            if (iu < n) __builtin_nontemporal_store(r[u], out + iu);
+        }
+    }
+}
+template __global__ void k_gather_lines<1>(const double*, const int32_t*, double*, uint64_t, const double*, uint64_t);
+template __global__ void k_gather_lines<2>(const double*, const int32_t*, double*, uint64_t, const double*, uint64_t);
 
 // Synthetic lookup inputs: E log-uniform in [emin, emax], material uniform.
 __global__ void k_xs_inputs(double* E, int32_t* mat, uint64_t n, double lnmin, double lnspan,

@@ -136,16 +136,6 @@ def test_free_gas_histories_bitexact(drain):
     np.testing.assert_allclose(g.sq, o.sq, rtol=1e-12, atol=1e-300)
 
 
-def test_error_paths_on_device():
-    # a copy ring that overflows (importance x8 per bin, 64 bank slots) ends the
-    # run with a capacity status rather than losing copies silently
-    from mc_testlib import importance_slab
-    p = mc.Problem.from_string(importance_slab(4.0, 0.5, 8, 2000, 8.0))
-    with pytest.raises(mc.MCError) as e:
-        mc.Engine(p, bank_capacity=64).forward()
-    assert e.value.status == mc.MC_ERR_CAPACITY
-
-
 def test_graph_replay_after_trace_count_changes():
     """A captured graph holds the trace record stride: a second forward with
     a smaller trace count on the same engine (two problems, graphs on) must

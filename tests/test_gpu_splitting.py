@@ -1,7 +1,7 @@
 """GPU importance splitting (population_control.cpp:21-49) against the oracle.
 
 Each particle's walk is bit-identical to the oracle's (copy i of a split
-starts (i + 1) 2^40 draws ahead of the parent's stream), so the total flight
+starts (i + 1) 2^50 draws ahead of the parent's stream), so the total flight
 count matches exactly.  A split family shares one tally row and its members
 add to it with atomics in whatever order they run, so per-history sums differ
 from the oracle's by rounding only: moment sums agree to 1e-12 relative.
@@ -158,3 +158,63 @@ def test_split_with_free_gas_and_energy_tally():
     p = mc.Problem.from_string(deck)
     assert p.n_bins > 2
     _check(p, n, bank_capacity=1 << 12)
+
+
+# Back-pressure of the copy ring: a split whose copies do not all fit leaves
+# its particle waiting at the surface (before its roulette) until they do, and
+# the engine doubles the ring once particles wait.  Importance x8 per bin makes
+# families of tens of thousands of members from one history; a bank of 256
+# slots starts with a ring of 2048 entries.
+@pytest.mark.parametrize("drain", [1, -1])
+def test_ring_back_pressure_matches_oracle(drain):
+    n = 300
+    p = mc.Problem.from_string(importance_slab(4.0, 0.5, 8, n, 8.0))
+    eng = mc.Engine(p, bank_capacity=256, drain_divisor=drain)
+    assert eng.copy_ring()["capacity"] == 0     # allocated by the first run
+    g = eng.forward()
+    o = oracle_run(p, n)
+    print(f"flights {o.flights} ring {eng.copy_ring()}", flush=True)
+    assert o.flights > 1000 * n                 # families far larger than the bank
+    assert g.flights == o.flights
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(g.sq, o.sq, rtol=1e-12, atol=1e-300)
+    r = eng.copy_ring()
+    assert r["waits"] > 0 and r["growths"] > 0 and r["capacity"] > 8 * 256
+    # the grown ring stays: the same run again needs fewer waits, same sums
+    g2 = eng.forward()
+    assert g2.flights == o.flights
+    np.testing.assert_allclose(g2.sum, o.sum, rtol=1e-12, atol=1e-300)
+    assert eng.copy_ring()["waits"] < r["waits"]
+
+
+def test_ring_back_pressure_gradient_matches_oracle():
+    """Gradient run whose importance rises x8 per bin with a 64-slot bank: a
+    waiting particle's copies carry its score function S as it was at the
+    split (nothing moves it while the particle waits)."""
+    n = 3000
+    deck = decks.slab_deck(2.0, 0.6, 4, n, params=("scatter", "capture"))
+    for k in range(1, 4):
+        deck = deck.replace(f'<cell name="bin{k}" material="slab">',
+                            f'<cell name="bin{k}" material="slab" importance="{8.0 ** k!r}">')
+    p = mc.Problem.from_string(deck)
+    eng = mc.Engine(p, bank_capacity=64)
+    go = np.zeros(p.n_bins)
+    go[0], go[1], go[2:] = 1.0, -0.5, 0.25
+    theta = np.array([1.05, 0.97])
+    g = eng.backward(go, theta=theta)
+    o = oracle_run(p, n, theta=theta, grad_out=go)
+    assert eng.copy_ring()["waits"] > 0
+    np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(g.grad, o.grad, rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(g.grad_sigma, o.grad_sigma, rtol=1e-8)
+    assert eng.forward(theta=theta).flights == o.flights
+
+
+def test_ring_does_not_grow_without_waits():
+    n = 4000
+    p = mc.Problem.from_string(importance_slab(4.0, 0.5, 8, n, 1.5))
+    eng = mc.Engine(p, bank_capacity=1 << 12)
+    g = eng.forward()
+    o = oracle_run(p, n)
+    assert g.flights == o.flights
+    assert eng.copy_ring() == {"capacity": 8 << 12, "growths": 0, "waits": 0}
