@@ -134,6 +134,12 @@ struct DParFold {
 // The same per nuclide entry, for materials where each nuclide has at most one
 // scatter-only and one capture-only parameter (mfold[m] = 1): problem-local
 // indices (-1: none) and 1/theta of the scatter parameter.
+// Score-function sums kept in LDS by the fused gradient flight kernel: for
+// materials of at most kAccNuc nuclides, kAccEntries doubles per thread of
+// dynamic LDS (the launch passes kAccEntries * 8 * blockDim bytes).
+constexpr int kAccNuc = 4;
+constexpr int kAccEntries = 2 * kAccNuc;
+
 struct DNucFold {
     int32_t kl_s, kl_c;
     double inv_theta_s;

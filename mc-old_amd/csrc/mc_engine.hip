@@ -881,13 +881,14 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     // every forward run, and gradient runs of several flights per iteration,
     // collide in the flight kernel
     const bool fused = !grad || A.events_per_launch > 1;
+    const size_t acc_lds = sizeof(double) * kAccEntries * b.x;   // the fused gradient kernel's S sums
     if (has_tables) {
         if (!grad) hipLaunchKernelGGL((k_flight<true, false, true>), g, b, 0, st, A);
-        else if (fused) hipLaunchKernelGGL((k_flight<true, true, true>), g, b, 0, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<true, true, true>), g, b, acc_lds, st, A);
         else hipLaunchKernelGGL((k_flight<true, true, false>), g, b, 0, st, A);
     } else {
         if (!grad) hipLaunchKernelGGL((k_flight<false, false, true>), g, b, 0, st, A);
-        else if (fused) hipLaunchKernelGGL((k_flight<false, true, true>), g, b, 0, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<false, true, true>), g, b, acc_lds, st, A);
         else hipLaunchKernelGGL((k_flight<false, true, false>), g, b, 0, st, A);
     }
     // gradient runs cross surfaces in a kernel of their own (timed with the flight)

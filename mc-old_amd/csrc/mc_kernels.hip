@@ -242,6 +242,12 @@ __device__ __forceinline__ LrPending lr_pending(const KArgs& A, uint32_t s)
     return LrPending{A.lpend[s], A.cs[s], A.cc[s], A.Qw[s]};
 }
 
+// The same without the flight length (lp = 0), for a caller that has it.
+__device__ __forceinline__ LrPending lr_pending_cs(const KArgs& A, uint32_t s)
+{
+    return LrPending{0.0, A.cs[s], A.cc[s], A.Qw[s]};
+}
+
 // Fold one parameter's pending terms into S and H: the scores' coefficients
 // cs, cc of dSigma_s/dtheta, dSigma_c/dtheta, then the increment
 // v = -lp dSigma_t/dtheta + extra of S and its H term v Q.  Terms that are
@@ -261,6 +267,48 @@ __device__ __forceinline__ void fold_one(const KArgs& A, uint32_t s, int kl, dou
     if (v != 0.0) {
         atomicAdd(&A.S[o], v);
         if (L.Q != 0.0) atomicAdd(&A.H[o], v * L.Q);
+    }
+}
+
+// Increments of S summed in LDS (the fused gradient flight kernel): a lane's
+// particle stays in one material from its first event of a launch to its
+// last (a surface crossing ends its launch), so the increments of that
+// material's parameters can be summed in the workgroup's LDS and added to S
+// once, when the particle stops, instead of once per collision.  Materials
+// of at most kAccNuc nuclides, each with at most one scatter and one capture
+// parameter (the folds' fast path); entry 2k (scatter) and 2k + 1 (capture)
+// of nuclide slot k, at acc[entry * blockDim.x] (consecutive lanes in
+// consecutive words).  H updates, rare (the particle has scored), stay
+// direct.  S then receives the same increments, summed in a different order
+// (rounding only).
+// fold_one with the increment of S added to the LDS entry *acc.
+__device__ __forceinline__ void fold_acc(const KArgs& A, uint32_t s, int kl, double ds, double dc, double extra,
+                                         const LrPending& L, double* acc)
+{
+    const size_t o = (size_t)kl * A.sstride + s;
+    if (L.cs != 0.0 || L.cc != 0.0) atomicAdd(&A.H[o], -(L.cs * ds + L.cc * dc));
+    double v = -L.lp * (ds + dc);
+    v += extra;
+    if (v != 0.0) {
+        // the entry is this lane's alone: an LDS add whose result is not used
+        __hip_atomic_fetch_add(acc, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (L.Q != 0.0) atomicAdd(&A.H[o], v * L.Q);
+    }
+}
+
+// Adds the lane's LDS sums for material m to S and clears them.
+__device__ __forceinline__ void flush_acc(const KArgs& A, uint32_t s, int m, double* acc)
+{
+    const mc_material_desc M = A.mat[m];
+    const uint32_t bd = blockDim.x;
+    for (int k = 0; k < M.nuc_count; ++k) {
+        const DNucFold F = A.nfold[M.nuc_begin + k];
+        double* e = acc + (size_t)(2 * k) * bd;
+        const double vs = e[0], vc = e[bd];
+        e[0] = 0.0;
+        e[bd] = 0.0;
+        if (F.kl_s >= 0 && vs != 0.0) atomicAdd(&A.S[(size_t)F.kl_s * A.sstride + s], vs);
+        if (F.kl_c >= 0 && vc != 0.0) atomicAdd(&A.S[(size_t)F.kl_c * A.sstride + s], vc);
     }
 }
 
@@ -1184,6 +1232,7 @@ template __global__ void k_resume<true>(KArgs);
 struct PState {
     Vec3 pos, dir;
     double E, w, sigt;
+    double lp;                    // gradient runs: the last flight's length, until its collision folds it
     uint64_t rng;
     int cell, prob, gidx;
     uint32_t nfl;
@@ -1201,6 +1250,7 @@ __device__ __forceinline__ PState load_state(const KArgs& A, uint32_t s, bool ta
     P.rng = A.rng[s];
     P.nfl = A.nflight[s];
     P.sigt = tables ? A.sigt[s] : 0.0;
+    P.lp = 0.0;                   // every launch's first event is a flight
     P.gidx = tables ? A.gidx[s] : 0;
     return P;
 }
@@ -1255,7 +1305,10 @@ __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& 
     // gradient runs: the flight's -l dSigma_t/dtheta is folded into S at the
     // collision or surface crossing that ends it, where the per-nuclide
     // cross sections at E are evaluated anyway
-    if (kGrad && C.material >= 0 && A.mat[C.material].param_count > 0) A.lpend[s] = l;
+    // (state in registers: a flight that ends in a collision hands its length
+    // to the collision in R; only one that ends on a surface stores it)
+    if (kGrad && kReg) R->lp = l;
+    if (kGrad && (!kReg || hit) && C.material >= 0 && A.mat[C.material].param_count > 0) A.lpend[s] = l;
     // move_particle, general.cpp:73-83
     p.x += u.x * l;
     p.y += u.y * l;
@@ -1324,7 +1377,8 @@ __device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
 // a wave's other lanes and its registers stay out of the flight loop.
 struct PState;
 template <bool kTables, bool kGrad, bool kInlineFreeGas = true, bool kReg = false>
-__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg = nullptr);
+__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg = nullptr,
+                                                 double* sacc = nullptr);
 
 template <bool kTables, bool kGrad, bool kFuse>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
@@ -1337,6 +1391,13 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     // them one iteration holds changes.  Without kFuse (gradient runs at one
     // flight per iteration) collisions go to queue C for k_collision.
     uint32_t extra = 0;                 // flights beyond the first, this thread
+    // fused gradient launches: kAccEntries doubles per thread of dynamic LDS,
+    // cleared here (each thread its own column)
+    extern __shared__ double lds_acc[];
+    if constexpr (kGrad && kFuse) {
+#pragma unroll
+        for (int e = 0; e < kAccEntries; ++e) lds_acc[e * blockDim.x + threadIdx.x] = 0.0;
+    }
     constexpr uint32_t kXq = 2048;      // >= the largest workgroup
     __shared__ uint32_t xq[kXq];
     __shared__ uint32_t xn, cn, sn;
@@ -1360,6 +1421,15 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                 // the state stays in registers from event to event
                 PState R = load_state(A, s, kTables);
                 const int m = kTables ? A.pmat[s] : -1;
+                // gradient runs: this lane's S sums in LDS, when its material allows
+                double* acc = nullptr;
+                if constexpr (kGrad) {
+                    const int ma = kTables ? m : A.cell[R.cell].material;
+                    if (ma >= 0 && A.mfold[ma]) {
+                        const mc_material_desc Ma = A.mat[ma];
+                        if (Ma.nuc_count <= kAccNuc) acc = lds_acc + threadIdx.x;
+                    }
+                }
                 for (;;) {
                     // the slot's field addresses are recomputed where they are
                     // used (one add each) rather than held across the events
@@ -1367,7 +1437,8 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                     st = flight_one<kTables, kGrad, true, true>(A, se, crossed, &R);
                     ++nfl;
                     if (st != ST_COLL) break;
-                    st = collision_one<kTables, kGrad, false, true>(A, se, &R);
+                    st = collision_one<kTables, kGrad, false, true>(A, se, &R, acc);
+                    if (kGrad && st != ST_ALIVE) acc = nullptr;     // its end flushed the sums
                     if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
                     // once fewer than a quarter of the wave's lanes are still
                     // flying, the rest stop too and fly on next iteration (the
@@ -1382,6 +1453,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                     }
                 }
                 const uint32_t so = opaque(s);
+                if (kGrad && acc) flush_acc(A, so, A.cell[R.cell].material, acc);
                 store_state(A, so, R, kTables);
                 A.status[so] = st;
             } else {
@@ -1488,7 +1560,7 @@ __device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
 // kReg: the particle's state is *reg, read and updated there; the returned
 // status is not stored (the caller stores the state and the status).
 template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg>
-__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg)
+__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg, double* sacc)
 {
     const int c = kReg ? reg->cell : A.pcell[s];
     const mc_cell_desc C = A.cell[c];
@@ -1504,7 +1576,7 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
         w = 0.0;
     } else {
         if (C.col_count > 0) {
-            const double lp = kGrad ? A.lpend[s] : 0.0;
+            const double lp = kGrad ? (kReg ? reg->lp : A.lpend[s]) : 0.0;
             for (int a = 0; a < C.col_count; ++a)
                 score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, lp, 0.0, false);
         }
@@ -1598,18 +1670,27 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
             // read again (the record or table lines are in cache).
             if (M.param_count > 0 && A.mfold[m]) {
                 // per nuclide: its scatter and capture parameters, sigma_s from registers
-                const LrPending L = lr_pending(A, s);
+                LrPending L = lr_pending_cs(A, s);
+                L.lp = kReg ? reg->lp : A.lpend[s];
 #pragma unroll
                 for (int k = 0; k < kNucReg; ++k) {
                     if (k < M.nuc_count) {
                         const int j = M.nuc_begin + k;
                         const DNucFold F = A.nfold[j];
                         const double N = A.mat_dens[j];
-                        if (F.kl_s >= 0)
-                            fold_one(A, s, F.kl_s, rs[k] * N, 0.0, sel == k ? F.inv_theta_s : 0.0, L);
+                        const double xtra = sel == k ? F.inv_theta_s : 0.0;
+                        if (F.kl_s >= 0) {
+                            if (kReg && sacc)   // (the material has at most kAccNuc nuclides)
+                                fold_acc(A, s, F.kl_s, rs[k] * N, 0.0, xtra, L, sacc + (size_t)(2 * k) * blockDim.x);
+                            else
+                                fold_one(A, s, F.kl_s, rs[k] * N, 0.0, xtra, L);
+                        }
                         if (F.kl_c >= 0) {
                             const double xs_c = rsc ? xs_pair(rsc + n2 + 2 * k, P) : entry_xs(A, j, XS_C, E, P, gm);
-                            fold_one(A, s, F.kl_c, 0.0, xs_c * N, 0.0, L);
+                            if (kReg && sacc)
+                                fold_acc(A, s, F.kl_c, 0.0, xs_c * N, 0.0, L, sacc + (size_t)(2 * k + 1) * blockDim.x);
+                            else
+                                fold_one(A, s, F.kl_c, 0.0, xs_c * N, 0.0, L);
                         }
                     }
                 }
@@ -1618,7 +1699,8 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
                     A.cc[s] = 0.0;
                 }
             } else if (M.param_count > 0) {
-                const LrPending L = lr_pending(A, s);
+                LrPending L = lr_pending_cs(A, s);
+                L.lp = kReg ? reg->lp : A.lpend[s];
                 for (int q = 0; q < M.param_count; ++q) {
                     const DParFold F = A.mpar[M.param_begin + q];
                     const int slot = F.slot, j = M.nuc_begin + slot;
@@ -1676,6 +1758,7 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
         reg->w = w;
         reg->rng = rng;
         if (alive) return ST_ALIVE;
+        if (kGrad && sacc) flush_acc(A, s, C.material, sacc);    // the closeout reads S
         return finish_history<kGrad, false>(A, s, p_id, PR, rng, reg->pos, E, reg->nfl);
     } else {
         A.u[s] = u.x; A.v[s] = u.y; A.w[s] = u.z;
