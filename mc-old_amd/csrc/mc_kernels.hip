@@ -275,10 +275,11 @@ __device__ __forceinline__ void fold_one(const KArgs& A, uint32_t s, int kl, dou
 // last (a surface crossing ends its launch), so the increments of that
 // material's parameters can be summed in the workgroup's LDS and added to S
 // once, when the particle stops, instead of once per collision.  Materials
-// of at most kAccNuc nuclides, each with at most one scatter and one capture
-// parameter (the folds' fast path); entry 2k (scatter) and 2k + 1 (capture)
-// of nuclide slot k, at acc[entry * blockDim.x] (consecutive lanes in
-// consecutive words).  H updates, rare (the particle has scored), stay
+// on the folds' fast path (each nuclide with at most one scatter and one
+// capture parameter) of at most kAccNuc nuclides: entry 2k (scatter) and
+// 2k + 1 (capture) of nuclide slot k; other materials of at most kAccEntries
+// parameters: entry q of their q-th parameter.  Entry e of a lane is
+// acc[e * blockDim.x] (consecutive lanes in consecutive words).  H updates, rare (the particle has scored), stay
 // direct.  S then receives the same increments, summed in a different order
 // (rounding only).
 // fold_one with the increment of S added to the LDS entry *acc.
@@ -296,11 +297,28 @@ __device__ __forceinline__ void fold_acc(const KArgs& A, uint32_t s, int kl, dou
     }
 }
 
+// Whether a lane in material m sums its S increments in LDS.
+__device__ __forceinline__ bool acc_material(const KArgs& A, int m)
+{
+    if (m < 0) return false;
+    const mc_material_desc M = A.mat[m];
+    return A.mfold[m] ? M.nuc_count <= kAccNuc : (M.param_count > 0 && M.param_count <= kAccEntries);
+}
+
 // Adds the lane's LDS sums for material m to S and clears them.
 __device__ __forceinline__ void flush_acc(const KArgs& A, uint32_t s, int m, double* acc)
 {
     const mc_material_desc M = A.mat[m];
     const uint32_t bd = blockDim.x;
+    if (!A.mfold[m]) {
+        for (int q = 0; q < M.param_count; ++q) {
+            double* e = acc + (size_t)q * bd;
+            const double v = e[0];
+            e[0] = 0.0;
+            if (v != 0.0) atomicAdd(&A.S[(size_t)A.mpar[M.param_begin + q].kl * A.sstride + s], v);
+        }
+        return;
+    }
     for (int k = 0; k < M.nuc_count; ++k) {
         const DNucFold F = A.nfold[M.nuc_begin + k];
         double* e = acc + (size_t)(2 * k) * bd;
@@ -1424,11 +1442,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                 // gradient runs: this lane's S sums in LDS, when its material allows
                 double* acc = nullptr;
                 if constexpr (kGrad) {
-                    const int ma = kTables ? m : A.cell[R.cell].material;
-                    if (ma >= 0 && A.mfold[ma]) {
-                        const mc_material_desc Ma = A.mat[ma];
-                        if (Ma.nuc_count <= kAccNuc) acc = lds_acc + threadIdx.x;
-                    }
+                    if (acc_material(A, kTables ? m : A.cell[R.cell].material)) acc = lds_acc + threadIdx.x;
                 }
                 for (;;) {
                     // the slot's field addresses are recomputed where they are
@@ -1721,7 +1735,10 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
                         dc = (rsc ? xs_pair(rsc + n2 + 2 * slot, P) : entry_xs(A, j, XS_C, E, P, gm)) * N;
                     const double extra =
                         (sel >= 0 && slot == sel && (F.reactions & MC_REACTION_SCATTER)) ? F.inv_theta : 0.0;
-                    fold_one(A, s, F.kl, ds, dc, extra, L);
+                    if (kReg && sacc)   // (the material has at most kAccEntries parameters)
+                        fold_acc(A, s, F.kl, ds, dc, extra, L, sacc + (size_t)q * blockDim.x);
+                    else
+                        fold_one(A, s, F.kl, ds, dc, extra, L);
                 }
                 if (L.cs != 0.0 || L.cc != 0.0) {
                     A.cs[s] = 0.0;
