@@ -167,6 +167,11 @@ typedef struct mc_problem_desc {
     double   wr, ws;                        /* weight roulette, simulator.h:90-91 */
     uint64_t n_histories;                   /* <description samples=...>          */
     uint64_t seed;                          /* LCG seed (default 1)               */
+    /* particle comb, <population_control><particle_comb bank_max teeth>
+     * (setup.cpp:57-67, population_control.cpp:55-84): after each particle's
+     * walk, a history whose bank of split copies holds bank_max or more is
+     * combed to comb_teeth copies.  0, 0: no comb.                         */
+    int32_t  comb_bank_max, comb_teeth;
 } mc_problem_desc;
 
 typedef struct mc_problem mc_problem;

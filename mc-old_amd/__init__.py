@@ -94,7 +94,8 @@ class ProblemDesc(C.Structure):
                                   "mat_density", "nuclides", "xs_energy", "xs_scatter", "xs_capture",
                                   "xs_total", "sources", "source_cdf", "estimators", "att_estimator",
                                   "att_index", "egrid", "params", "mat_param")] + [
-        ("wr", C.c_double), ("ws", C.c_double), ("n_histories", C.c_uint64), ("seed", C.c_uint64)]
+        ("wr", C.c_double), ("ws", C.c_double), ("n_histories", C.c_uint64), ("seed", C.c_uint64),
+        ("comb_bank_max", C.c_int32), ("comb_teeth", C.c_int32)]
 
 
 EXPORTS = {

@@ -29,7 +29,14 @@ constexpr int kBucketShift = 52 - kBucketBits;
 // copies, which score into their own family's row, may take the slot).
 // ST_WAIT: the particle split at a surface and waits there for room in the
 // copy ring for the rest of its copies (k_resume retries each iteration).
-enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4, ST_HELD = 5, ST_WAIT = 6 };
+// ST_POP: the particle of a history with a particle comb ended while copies
+// wait on its slot's own bank; k_resume combs the bank and starts the top copy.
+enum : uint8_t { ST_EMPTY = 0, ST_ALIVE = 1, ST_DEAD = 2, ST_SURF = 3, ST_COLL = 4, ST_HELD = 5, ST_WAIT = 6,
+                 ST_POP = 7 };
+
+// Entries a history's bank may gain from one particle's walk beyond the
+// comb's bank_max (its capacity is max(bank_max - 1 + kCombSlack, teeth)).
+constexpr int kCombSlack = 512;
 
 constexpr int kTouched = 8;        // per-slot list of tally bins touched by the history
 
@@ -80,6 +87,7 @@ struct Counters {
     uint32_t cplaced, pad2;       // copies the last refill placed
     uint64_t xfl;                 // flights taken beyond the first of an iteration (included in tA)
     uint64_t waits;               // splits and turns that found the ring without room
+    uint32_t err_bank, pad3;      // split copies a history's comb bank had no room for
 };
 
 // Entry i of queue A.
@@ -230,6 +238,14 @@ struct KArgs {
     // a particle whose split copies did not all fit in the ring waits at the
     // surface: ppend copies still to place, the next being copy pfirst
     int32_t *ppend, *pfirst;
+    // problems with a particle comb: per slot a LIFO bank of its history's
+    // split copies (entry i of slot s at s * bcap + i; bsize entries), run one
+    // after another in the slot as the reference runs its bank
+    double *bx, *by, *bz, *bu, *bv, *bw, *bE, *bwgt;
+    uint64_t* brng;
+    int32_t *bcell, *bsize;
+    uint32_t bcap;
+    const int32_t* comb;          // per problem (bank_max, teeth), (0, 0): no comb; null: no problem has one
     uint32_t* wg_held;            // per workgroup: ended slots whose row a family holds
     // copy ring (capacity ccap): particles made by importance splitting
     double *cx, *cy, *cz, *cu, *cv, *cw, *cE, *cwgt;

@@ -159,6 +159,9 @@ struct Engine::Impl {
     int n_bins = 0, n_params = 0, bin_stride = 1, par_stride = 1;
     bool has_tables = false;
     bool split_any = false;            // some problem's cells differ in importance
+    bool comb_any = false;             // some problem that can split has a particle comb
+    uint64_t comb_cap = 0;             // entries of a slot's comb bank
+    std::vector<int32_t> comb_par;     // per problem (bank_max, teeth)
     double emin = 0, emax = 0;
 
     // device data
@@ -173,6 +176,7 @@ struct Engine::Impl {
         b_status, b_tcount, b_tbins, b_hist, b_gidx, b_S, b_H, b_Qw, b_cs, b_cc, b_lp, b_vmat, q_S, q_A, q_C, d_ctr, d_wgcnt, b_mat,
         d_wgxfl;
     // split families and the copy ring (problems that can split only)
+    DevBuf c_x, c_y, c_z, c_u, c_v, c_w, c_E, c_wgt, c_rng, c_cell, c_size, d_comb;   // comb banks
     DevBuf b_fam, b_fcount, b_ppend, b_pfirst, d_wgheld, r_S, r_vmat, b_famG, r_x, r_y, r_z, r_u, r_v, r_w, r_E, r_wgt, r_rng, r_hid, r_cell, r_prob, r_fam,
         r_nfl;
     uint64_t ring_cap = 0;
@@ -321,7 +325,15 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
             if (I > 0.0 && (imin == 0.0 || I < imin)) imin = I;
             imax = std::max(imax, I);
         }
-        if (imin > 0.0 && imax > imin) split_any = true;
+        if (imin > 0.0 && imax > imin) {
+            split_any = true;
+            if (d.comb_teeth > 0) {
+                // a bank holds up to bank_max - 1 copies plus what one walk adds
+                comb_any = true;
+                comb_cap = std::max<uint64_t>(comb_cap, std::max<uint64_t>((uint64_t)d.comb_bank_max - 1 + kCombSlack,
+                                                                           (uint64_t)d.comb_teeth));
+            }
+        }
         for (int c = 0; c < d.n_cells; ++c) {
             mc_cell_desc C = d.cells[c];
             if (C.material >= 0) C.material += mat_o;
@@ -397,6 +409,8 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         R.mat_begin = mat_o;
         R.mat_count = d.n_materials;
         R.seed = d.seed;
+        comb_par.push_back(d.comb_bank_max);
+        comb_par.push_back(d.comb_teeth);
         R.wr = d.wr;
         R.ws = d.ws;
         prob.push_back(R);
@@ -581,6 +595,7 @@ void Engine::Impl::upload_static()
     upload(d_src_cdf, src_cdf, stream);
     upload(d_cdf_off, cdf_off, stream);
     upload(d_jump, jump, stream);
+    if (comb_any) upload(d_comb, comb_par, stream);
     if (split_any) d_wgheld.alloc(sizeof(uint32_t) * 8192);
     d_run_hb.alloc(sizeof(uint64_t) * (prob.size() + 1));
     d_run_jd.alloc(sizeof(uint64_t) * prob.size());
@@ -656,9 +671,17 @@ bool Engine::Impl::grow_ring(uint64_t head, uint64_t tail)
 // power of two apart (a fold updates up to eight of them at once).
 inline uint64_t lr_stride(uint64_t cap) { return cap + 4128; }
 
+// Device memory the comb banks of one engine may take.
+constexpr uint64_t kCombBankBytes = 16ull << 30;
+
 void Engine::Impl::ensure_bank(bool grad)
 {
-    const uint64_t B = cfg.bank_capacity;
+    uint64_t B = cfg.bank_capacity;
+    if (comb_any) {
+        // every slot carries a bank of comb_cap copies (76 B each)
+        const uint64_t fit = kCombBankBytes / (comb_cap * 76ull) / 1024 * 1024;
+        B = std::max<uint64_t>(1024, std::min(B, fit));
+    }
     if (bank_cap == B && (!grad || bank_grad)) return;
     const bool realloc_all = bank_cap != B;
     if (realloc_all) {
@@ -684,6 +707,12 @@ void Engine::Impl::ensure_bank(bool grad)
             b_fcount.alloc(4 * B);
             b_ppend.alloc(4 * B);
             b_pfirst.alloc(4 * B);
+            if (comb_any) {
+                const uint64_t R = B * comb_cap;
+                for (DevBuf* c : {&c_x, &c_y, &c_z, &c_u, &c_v, &c_w, &c_E, &c_wgt, &c_rng}) c->alloc(8 * R);
+                c_cell.alloc(4 * R);
+                c_size.alloc(4 * B);
+            }
             for (DevBuf* r : {&r_x, &r_y, &r_z, &r_u, &r_v, &r_w, &r_E, &r_wgt, &r_rng, &r_hid}) r->alloc(8 * R);
             for (DevBuf* r : {&r_cell, &r_prob, &r_fam, &r_nfl}) r->alloc(4 * R);
             ring_cap = R;
@@ -782,6 +811,17 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
         A.fcount = b_fcount.as<int32_t>();
         A.ppend = b_ppend.as<int32_t>();
         A.pfirst = b_pfirst.as<int32_t>();
+        if (comb_any) {
+            A.bx = c_x.as<double>(); A.by = c_y.as<double>(); A.bz = c_z.as<double>();
+            A.bu = c_u.as<double>(); A.bv = c_v.as<double>(); A.bw = c_w.as<double>();
+            A.bE = c_E.as<double>();
+            A.bwgt = c_wgt.as<double>();
+            A.brng = c_rng.as<uint64_t>();
+            A.bcell = c_cell.as<int32_t>();
+            A.bsize = c_size.as<int32_t>();
+            A.bcap = (uint32_t)comb_cap;
+            A.comb = d_comb.as<int32_t>();
+        }
         A.wg_held = d_wgheld.as<uint32_t>();
         A.cx = r_x.as<double>(); A.cy = r_y.as<double>(); A.cz = r_z.as<double>();
         A.cu = r_u.as<double>(); A.cv = r_v.as<double>(); A.cw = r_w.as<double>();
@@ -1062,6 +1102,9 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
                        double* grad_mean, double* grad_sigma)
 {
     HIP_CHECK(hipSetDevice(device));
+    if (grad && comb_any)
+        throw Error(MC_ERR_UNSUPPORTED, "gradients of problems with a particle comb: the comb's selection "
+                                        "of copies has no likelihood-ratio term here");
     cur = (run && run->stream) ? (hipStream_t)run->stream : stream;
     const bool dev = run && run->device_ptrs;
     const int np = (int)prob.size();
@@ -1222,7 +1265,7 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     HIP_CHECK(hipEventRecord(ev0, cur));
     const bool plain = run && run->mode == MC_RUN_PLAIN;
     uint64_t flights = 0;
-    uint32_t err_lost = 0, err_split = 0;
+    uint32_t err_lost = 0, err_split = 0, err_bank = 0;
     uint64_t waits = 0;
     for (int p = 0; p < (plain ? np : 1); ++p) {
         Counters c{};
@@ -1238,10 +1281,12 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
             hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.fam, bank_cap, -1);
             hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.fcount, bank_cap, 0);
             hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.ppend, bank_cap, 0);
+            if (comb_any) hipLaunchKernelGGL(k_fill_i32, fg, fb, 0, cur, A.bsize, bank_cap, 0);
         }
         run_loop(grad, A);
         flights += h_ctr->tA;      // every entry of queue A is one flight
         waits += h_ctr->waits;
+        err_bank += h_ctr->err_bank;
         err_lost += h_ctr->err_lost;
         err_split += h_ctr->err_split;
     }
@@ -1250,6 +1295,10 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
     float ms = 0.f;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
     last_waits = waits;
+    if (err_bank)
+        throw Error(MC_ERR_CAPACITY, std::to_string(err_bank) + " split copies found a history's comb bank full (" +
+                                         std::to_string(comb_cap) + " entries: bank_max - 1 + " +
+                                         std::to_string(kCombSlack) + ")");
     if (err_lost) throw Error(MC_ERR_LOST_PARTICLE, std::to_string(err_lost) + " particle(s) lost");
     if (err_split)
         throw Error(MC_ERR_ARGUMENT, "a particle split in a problem without an importance rise");

@@ -541,6 +541,11 @@ template <bool kGrad, bool kFlush>
 __device__ __forceinline__ uint8_t finish_history(const KArgs& A, uint32_t s, int p, const DProb& PR, uint64_t rng, Vec3 pos,
                                   double E, uint32_t nfl)
 {
+    if constexpr (!kGrad) {
+        // particle comb: the history goes on with the next copy of its bank
+        // (only a comb problem's particles push copies on their slot's bank)
+        if (A.bsize && A.bsize[s] > 0) return ST_POP;
+    }
     if (A.split_any && A.fam[s] >= 0) {
         if (kGrad) grad_closeout<kFlush>(A, s, PR, E, true, A.fam[s]);
         return finish_family<kGrad>(A, s, PR);
@@ -1106,6 +1111,110 @@ __device__ __forceinline__ uint32_t spawn_copies(const KArgs& A, uint32_t s, int
     return nc - push_copies<kGrad>(A, s, 0, nc, p, u, E, w, rng, nfl, cell, p_id);
 }
 
+// Problems with a particle comb keep a history's split copies on its slot's
+// own bank (handler.cpp:19-29: the reference's Pbank, last in first out):
+// copies first .. first + count - 1 of a split go on top in order, copy i
+// starting (i + 1) 2^50 draws ahead of the particle's stream as ring copies
+// do.  A full bank drops the copy and counts it (the run then fails).
+__device__ __forceinline__ void push_bank(const KArgs& A, uint32_t s, uint32_t first, uint32_t count, Vec3 p, Vec3 u,
+                                          double E, double w, uint64_t rng, int cell)
+{
+    int b = A.bsize[s];
+    uint64_t r = rng;
+    for (uint32_t i = 0; i < first; ++i) r = (r * A.jump_copy) & kRnMask;
+    for (uint32_t i = 0; i < count; ++i) {
+        r = (r * A.jump_copy) & kRnMask;
+        if ((uint32_t)b >= A.bcap) {
+            atomicAdd(&A.ctr->err_bank, 1u);
+            continue;
+        }
+        const size_t k = (size_t)s * A.bcap + (uint32_t)b;
+        A.bx[k] = p.x; A.by[k] = p.y; A.bz[k] = p.z;
+        A.bu[k] = u.x; A.bv[k] = u.y; A.bw[k] = u.z;
+        A.bE[k] = E;
+        A.bwgt[k] = w;
+        A.brng[k] = r;
+        A.bcell[k] = cell;
+        ++b;
+    }
+    A.bsize[s] = b;
+}
+
+// The end of one particle of a comb history (status ST_POP): Simulator::
+// particle_comb (population_control.cpp:55-84) on the slot's bank, then the
+// top copy starts in the slot (handler.cpp:23).  A bank of bank_max or more
+// is combed to teeth copies: with W its total weight and w_avg =
+// W / teeth, a tooth at u w_avg, u the next draw of the ended particle's
+// stream (the reference draws from its one global stream), then every
+// w_avg; walking the bank in order, a copy whose running weight sum passes
+// the tooth is kept at weight w_avg and the tooth moves on (once per copy).
+// Entries past the last kept one are the bank's first entry as it was, its
+// weight included (the reference fills them from Pbank[0]); the k-th of
+// them (from 1) starts k 2^50 draws ahead of that entry's stream, so no two
+// entries share one.  The popped copy keeps the slot's history, tally row
+// and flight count.
+__device__ __forceinline__ void comb_pop(const KArgs& A, uint32_t s)
+{
+    const int p = A.pprob[s];
+    const size_t base = (size_t)s * A.bcap;
+    int n = A.bsize[s];
+    if (n >= A.comb[2 * p]) {
+        const int teeth = A.comb[2 * p + 1];
+        uint64_t rng = A.rng[s];
+        double W = 0.0;
+        for (int i = 0; i < n; ++i) W += A.bwgt[base + i];
+        const double w_avg = W / (double)teeth;
+        double tooth = urand(rng) * w_avg;
+        const double x0 = A.bx[base], y0 = A.by[base], z0 = A.bz[base];
+        const double u0 = A.bu[base], v0 = A.bv[base], ww0 = A.bw[base];
+        const double E0 = A.bE[base], w0 = A.bwgt[base];
+        const uint64_t r0 = A.brng[base];
+        const int c0 = A.bcell[base];
+        double sum = 0.0;
+        int j = 0;
+        for (int i = 0; i < n && j < teeth; ++i) {     // j <= i: kept copies move down in place
+            sum += A.bwgt[base + i];
+            if (sum > tooth) {
+                const size_t a = base + i, b = base + j;
+                if (a != b) {
+                    A.bx[b] = A.bx[a]; A.by[b] = A.by[a]; A.bz[b] = A.bz[a];
+                    A.bu[b] = A.bu[a]; A.bv[b] = A.bv[a]; A.bw[b] = A.bw[a];
+                    A.bE[b] = A.bE[a];
+                    A.brng[b] = A.brng[a];
+                    A.bcell[b] = A.bcell[a];
+                }
+                A.bwgt[b] = w_avg;
+                tooth += w_avg;
+                ++j;
+            }
+        }
+        uint64_t r = r0;
+        for (int k = j; k < teeth; ++k) {
+            r = (r * A.jump_copy) & kRnMask;
+            const size_t b = base + k;
+            A.bx[b] = x0; A.by[b] = y0; A.bz[b] = z0;
+            A.bu[b] = u0; A.bv[b] = v0; A.bw[b] = ww0;
+            A.bE[b] = E0;
+            A.bwgt[b] = w0;
+            A.brng[b] = r;
+            A.bcell[b] = c0;
+        }
+        n = teeth;
+    }
+    --n;
+    const size_t k = base + n;
+    const int c = A.bcell[k];
+    A.x[s] = A.bx[k]; A.y[s] = A.by[k]; A.z[s] = A.bz[k];
+    A.u[s] = A.bu[k]; A.v[s] = A.bv[k]; A.w[s] = A.bw[k];
+    A.E[s] = A.bE[k];
+    A.wgt[s] = A.bwgt[k];
+    A.rng[s] = A.brng[k];
+    A.pcell[s] = c;
+    A.pmat[s] = A.cell[c].material;
+    A.bsize[s] = n;
+    A.status[s] = ST_ALIVE;
+}
+
 template <bool kGrad>
 __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, int sf, Vec3 p, Vec3 u, double E,
                                                  double w, uint64_t rng, uint32_t nfl, int cell_old, int p_id)
@@ -1161,13 +1270,17 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
             const int nsplit = (int)floor(rat + urand(rng));
             w = w / (double)nsplit;
             if (nsplit > 1) {
-                const uint32_t left = spawn_copies<kGrad>(A, s, nsplit, p, u, E, w, rng, nfl, cell, p_id);
+                // comb problems: the copies go on the slot's own bank, at the
+                // next k_resume, while the particle waits here
+                const bool comb = !kGrad && A.comb && A.comb[2 * p_id + 1] > 0;
+                const uint32_t left =
+                    comb ? (uint32_t)(nsplit - 1) : spawn_copies<kGrad>(A, s, nsplit, p, u, E, w, rng, nfl, cell, p_id);
                 if (left > 0) {
                     // the particle waits here, before its roulette, with the
                     // state its remaining copies are made from
                     A.ppend[s] = (int32_t)left;
                     A.pfirst[s] = nsplit - 1 - (int32_t)left;
-                    atomicAdd((unsigned long long*)&A.ctr->waits, 1ull);
+                    if (!comb) atomicAdd((unsigned long long*)&A.ctr->waits, 1ull);
                     waiting = true;
                 }
             }
@@ -1192,8 +1305,9 @@ __device__ __forceinline__ uint8_t surface_crossing(const KArgs& A, uint32_t s, 
 }
 
 // A turn of a particle waiting at a surface (status ST_WAIT): it places the
-// copies the ring has room for and, once all are placed, takes the roulette
-// it stopped before (surface_crossing).  Returns ST_WAIT while copies are
+// copies the ring has room for (a comb problem's: all of them, on its slot's
+// bank) and, once all are placed, takes the roulette it stopped before
+// (surface_crossing).  Returns ST_WAIT while copies are
 // left, else ST_ALIVE (it flies on) or the status its end gives the slot.
 template <bool kGrad>
 __device__ __forceinline__ uint8_t resume_parked(const KArgs& A, uint32_t s)
@@ -1206,7 +1320,14 @@ __device__ __forceinline__ uint8_t resume_parked(const KArgs& A, uint32_t s)
     uint64_t rng = A.rng[s];
     const uint32_t nfl = A.nflight[s];
     const int cell = A.pcell[s], p_id = A.pprob[s];
-    const uint32_t put = push_copies<kGrad>(A, s, first, left, p, u, E, w, rng, nfl, cell, p_id);
+    const DProb PR = A.prob[p_id];
+    uint32_t put;
+    if (!kGrad && A.comb && A.comb[2 * p_id + 1] > 0) {
+        push_bank(A, s, first, left, p, u, E, w, rng, cell);     // a comb problem's own bank
+        put = left;
+    } else {
+        put = push_copies<kGrad>(A, s, first, left, p, u, E, w, rng, nfl, cell, p_id);
+    }
     if (put < left) {
         A.ppend[s] = (int32_t)(left - put);
         A.pfirst[s] = (int32_t)(first + put);
@@ -1214,7 +1335,6 @@ __device__ __forceinline__ uint8_t resume_parked(const KArgs& A, uint32_t s)
         return ST_WAIT;
     }
     A.ppend[s] = 0;
-    const DProb PR = A.prob[p_id];
     bool alive = true;
     weight_roulette(w, alive, rng, PR);
     A.wgt[s] = w;
@@ -1224,15 +1344,28 @@ __device__ __forceinline__ uint8_t resume_parked(const KArgs& A, uint32_t s)
 }
 
 // First kernel of an iteration of a run that can split: waiting particles
-// retry before the refill counts the slots.
+// retry, and comb histories start their next copy, before the refill counts
+// the slots.
 template <bool kGrad>
 __global__ void __launch_bounds__(1024) k_resume(KArgs A)
 {
     uint32_t lo, hi;
     block_range(A.ctr->active_cap, lo, hi);
     for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        if (A.status[i] != ST_WAIT) continue;
+        const uint8_t s0 = A.status[i];
+        if (!kGrad && s0 == ST_POP) {
+            comb_pop(A, i);
+            continue;
+        }
+        if (s0 != ST_WAIT) continue;
         const uint8_t st = resume_parked<kGrad>(A, i);
+        if (!kGrad && st == ST_POP) {
+            // its roulette ended it and its bank holds copies: the next one
+            // starts now, so that no slot is left between statuses when the
+            // refill counts live particles
+            comb_pop(A, i);
+            continue;
+        }
         if (st != ST_WAIT) A.status[i] = st;
     }
 }

@@ -73,6 +73,8 @@ void Problem::finalize()
     desc.ws = ws;
     desc.n_histories = n_histories;
     desc.seed = seed;
+    desc.comb_bank_max = comb_bank_max;
+    desc.comb_teeth = comb_teeth;
 }
 
 static void need_finite(const std::vector<double>& E, const std::string& path)
@@ -175,8 +177,13 @@ void load_problem_xml(Problem& P, const std::string& xml, const std::string& xs_
     need(!sim->child("ksearch"), MC_ERR_UNSUPPORTED, "k-eigenvalue search is out of scope (non-multiplying media only)");
     need(!sim->child("tdmc"), MC_ERR_UNSUPPORTED, "time-dependent MC is out of scope");
     need(!sim->child("trmm"), MC_ERR_UNSUPPORTED, "TRMM is out of scope");
-    if (const XmlNode* pc = top("population_control")) {
-        need(!pc->child("particle_comb"), MC_ERR_UNSUPPORTED, "particle comb is not supported");
+    if (const XmlNode* pc = top("population_control")) {                 // setup.cpp:57-67
+        if (const XmlNode* comb = pc->child("particle_comb")) {
+            P.comb_bank_max = comb->as_int("bank_max");
+            P.comb_teeth = comb->as_int("teeth");
+            need(P.comb_bank_max >= 1 && P.comb_teeth >= 1, MC_ERR_INPUT,
+                 "<particle_comb> needs bank_max >= 1 and teeth >= 1");
+        }
     }
 
     // ------------------------------------------------------- distributions
@@ -727,6 +734,10 @@ void load_problem_desc(Problem& P, const mc_problem_desc& d)
     need(P.n_histories > 1, MC_ERR_INPUT, "mc_problem_desc.n_histories must be at least 2");
     P.seed = d.seed;
     need(P.seed > 0 && (P.seed & 1) == 1, MC_ERR_INPUT, "mc_problem_desc.seed must be a positive odd integer");
+    P.comb_bank_max = d.comb_bank_max;
+    P.comb_teeth = d.comb_teeth;
+    need((P.comb_bank_max == 0 && P.comb_teeth == 0) || (P.comb_bank_max >= 1 && P.comb_teeth >= 1), MC_ERR_INPUT,
+         "mc_problem_desc: comb_bank_max and comb_teeth are both 0 (no comb) or both at least 1");
     // names by position; report metadata from the records
     P.name = "problem";
     for (size_t k = 0; k < P.surfaces.size(); ++k) P.surface_names.push_back("surface " + std::to_string(k));

@@ -41,6 +41,7 @@ struct Problem {
     double wr = 0.001, ws = 1.0;       // simulator.h:90-91
     uint64_t n_histories = 0;
     uint64_t seed = 1;                  // Random.cpp:105
+    int32_t comb_bank_max = 0, comb_teeth = 0;   // particle comb (0: off), setup.cpp:57-67
     int32_t n_bins = 0;
 
     mc_problem_desc desc{};

@@ -16,7 +16,10 @@
  *   - every history owns an LCG stream started by RN_init_particle
  *     (Random.cpp:196-204) at key = history index, instead of one global
  *     stream; a split copy's stream starts 2^50 draws per copy ahead of its
- *     parent's (see cell_importance).
+ *     parent's (see cell_importance);
+ *   - the particle comb draws its tooth from the stream of the particle
+ *     whose walk just ended, and its fill entries get jumped streams (see
+ *     particle_comb).
  *
  * Build: gcc -O2 -ffp-contract=off (no FMA contraction: bit-exactness).
  */
@@ -608,6 +611,70 @@ static void cell_importance(ostate* st, opart* P)
     }
 }
 
+/* Particle comb, population_control.cpp:55-84, after each particle's walk
+ * (handler.cpp:24-28): a bank of bank_max or more copies is combed to
+ * comb_teeth, each selected copy taking the average weight; the selection
+ * walks the bank in order with one tooth per selected copy.  Entries past
+ * the last selected one stay copies of the bank's first entry, as the
+ * reference fills them (bank_tmp(comb_teeth, Pbank[0])), weight included.
+ * The tooth's random number is the next draw of the stream of the particle
+ * whose walk just ended (the reference draws it from its one global
+ * stream), and the fill copies' streams are jumped like split copies' (k-th
+ * fill: (k + 1) 2^50 draws ahead of the first entry's), so that no two
+ * entries walk the same stream.  No comb in gradient runs.                 */
+int mco_comb_plan(const double* w, int n, int teeth, double u, int* src, double* wout, int* fill)
+{
+    double W = 0.0;
+    for (int i = 0; i < n; ++i) W += w[i];
+    const double w_avg = W / (double)teeth;
+    double tooth = u * w_avg;
+    double sum = 0.0;
+    int j = 0;
+    /* j < teeth: the last tooth lies below W by u w_avg; a first tooth within
+     * rounding of 0 could otherwise select one entry more than the reference's
+     * bank_tmp holds */
+    for (int i = 0; i < n && j < teeth; ++i) {
+        sum += w[i];
+        if (sum > tooth) {
+            src[j] = i;
+            wout[j] = w_avg;
+            fill[j] = 0;
+            tooth += w_avg;
+            ++j;
+        }
+    }
+    const int selected = j;
+    for (int k = selected; k < teeth; ++k) {
+        src[k] = 0;
+        wout[k] = w[0];
+        fill[k] = k - selected + 1;
+    }
+    return selected;
+}
+
+static void particle_comb(ostate* st, uint64_t* rng)
+{
+    const int bank_max = st->d->comb_bank_max, teeth = st->d->comb_teeth;
+    if (bank_max <= 0 || st->bank_n < bank_max) return;
+    const int n = st->bank_n;
+    double* w = (double*)malloc(sizeof(double) * (size_t)n);
+    double* wout = (double*)malloc(sizeof(double) * (size_t)teeth);
+    int* src = (int*)malloc(sizeof(int) * (size_t)teeth);
+    int* fill = (int*)malloc(sizeof(int) * (size_t)teeth);
+    for (int i = 0; i < n; ++i) w[i] = st->bank[i].w;
+    mco_comb_plan(w, n, teeth, mco_urand(rng), src, wout, fill);
+    opart* out = (opart*)malloc(sizeof(opart) * (size_t)teeth);
+    for (int k = 0; k < teeth; ++k) {
+        out[k] = st->bank[src[k]];
+        out[k].w = wout[k];
+        if (fill[k]) out[k].rng = mco_skip_ahead(st->bank[0].rng, (int64_t)fill[k] << MCO_COPY_JUMP_LOG2);
+    }
+    free(st->bank);
+    st->bank = out;
+    st->bank_n = st->bank_cap = teeth;
+    free(w); free(wout); free(src); free(fill);
+}
+
 /* Fold the pending terms of the current material at E into S and H: the
  * scores' coefficients cs, cc, then the flight's -lpend dSigma_t/dtheta plus
  * extra[q] (the nuclide-selection term at a collision).                    */
@@ -857,6 +924,7 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
             const double* grad_out, double* gsum, double* gsq)
 {
     if (!d || !sum || !sq) return MC_ERR_ARGUMENT;
+    if (d->comb_bank_max > 0 && grad_out) return MC_ERR_UNSUPPORTED;   /* no comb in gradient runs */
     ostate st;
     memset(&st, 0, sizeof(st));
     st.d = d;
@@ -907,10 +975,12 @@ int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
         const uint64_t f0 = st.flights;
         random_walk(&st, &P);
         opart last = P;
+        particle_comb(&st, &P.rng);
         while (st.bank_n > 0 && !st.error) {
             opart Q = st.bank[--st.bank_n];
             random_walk(&st, &Q);
             last = Q;
+            particle_comb(&st, &Q.rng);
             if (Q.S) { free(Q.S); free(Q.H); }
         }
         /* Estimator::end_history, Estimator.cpp:339-346 */

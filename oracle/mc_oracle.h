@@ -67,6 +67,13 @@ double mco_material_xs(const mc_problem_desc* d, const double* theta, int mat, i
  * grad_out (optional, n_bins) switches on the likelihood-ratio gradient:
  * gsum/gsq (n_params) accumulate the per-history vector-Jacobian product.
  * Returns MC_OK or an mc_status error.                                       */
+/* The particle comb's selection (population_control.cpp:55-84) over bank
+ * weights w[0..n) with first tooth u * W / teeth: output entry k is a copy of
+ * entry src[k] with weight wout[k]; fill[k] > 0 marks the k-th entry past the
+ * last selected one (a copy of entry 0 at its own weight, whose stream the
+ * comb jumps fill[k] 2^50 draws ahead).  Returns the number selected.       */
+int mco_comb_plan(const double* w, int n, int teeth, double u, int* src, double* wout, int* fill);
+
 int mco_run(const mc_problem_desc* d, const double* theta, uint64_t key_offset,
             uint64_t n_hist, double* sum, double* sq, uint64_t* flights,
             mc_trace_record* trace, uint64_t trace_count,
