@@ -86,10 +86,14 @@ def test_batched_slabs_events_per_launch(k):
 def test_splitting_with_several_events_per_launch():
     """Importance doubling per bin: a run that can split takes one flight per
     particle per iteration whatever the setting (its copies wait in the ring
-    for free slots, which appear once per iteration)."""
+    for free slots, which appear once per iteration).  c = 0.7 with 4096
+    slots: large families in a small bank (the copy ring applies
+    back-pressure when they outgrow it)."""
     n = 6000
-    p = mc.Problem.from_string(importance_slab(4.0, 0.5, 8, n, 2.0))
-    g = mc.Engine(p, bank_capacity=1 << 14, events_per_launch=5).forward()
+    p = mc.Problem.from_string(importance_slab(4.0, 0.7, 8, n, 2.0))
+    eng = mc.Engine(p, bank_capacity=1 << 12, events_per_launch=5)
+    g = eng.forward()
+    print("copy ring", eng.copy_ring(), flush=True)
     o = oracle_run(p, n)
     assert g.flights == o.flights
     np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12, atol=1e-300)
