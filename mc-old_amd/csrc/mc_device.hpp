@@ -541,17 +541,19 @@ __device__ __forceinline__ double material_sigt(const KArgs& A, int m, double E,
 // Sigma_t of an unscaled material on the LDS grid: the bucket from LDS, then
 // the record of the interval's lower point, which carries the grid energies
 // the choice and the interpolation need.  Same interval, same values, same
-// arithmetic as material_sigt.
-__device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLook& L, const uint16_t* sb, double E,
-                                                   int& idx)
+// arithmetic as material_sigt.  key_base, te_off, te_stride, n, nb and
+// nuc_count are the material's DMatLook fields, dens its nuclide densities.
+__device__ __forceinline__ double sigt_lds_core(const KArgs& A, int64_t key_base, int64_t te_off, int te_stride,
+                                                int n, int nb, int nuc_count, const double* dens,
+                                                const uint16_t* sb, double E, int& idx)
 {
-    const int64_t key = (int64_t)(double_to_bits(E) >> kBucketShift) - L.key_base;
-    const double* base = A.rec_te + L.te_off;
-    const int64_t st = L.te_stride;
-    const int last = L.n - 1;
+    const int64_t key = (int64_t)(double_to_bits(E) >> kBucketShift) - key_base;
+    const double* base = A.rec_te + te_off;
+    const int64_t st = te_stride;
+    const int last = n - 1;
     double a[kTeVals], b[kTeVals];       // records i and i + 1 of the interval
     int i;
-    if (key < 0 || key >= L.nb) {
+    if (key < 0 || key >= nb) {
         i = key < 0 ? -1 : last;
         const double* r = base + (i > 0 ? i : 0) * st;
 #pragma unroll
@@ -582,10 +584,28 @@ __device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLo
     double t = 0.0;
 #pragma unroll
     for (int k = 0; k < kFastNuc; ++k)
-        if (k < L.nuc_count) t += (interp ? pa * a[1 + k] + pb * b[1 + k] : a[1 + k]) * A.mat_dens[L.nuc_begin + k];
+        if (k < nuc_count) t += (interp ? pa * a[1 + k] + pb * b[1 + k] : a[1 + k]) * dens[k];
     idx = i;
     return t;
 }
+
+__device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLook& L, const uint16_t* sb, double E,
+                                                   int& idx)
+{
+    return sigt_lds_core(A, L.key_base, L.te_off, L.te_stride, L.n, L.nb, L.nuc_count, A.mat_dens + L.nuc_begin, sb,
+                         E, idx);
+}
+
+// The standalone lookup's copy in LDS of what its path reads per material
+// (the DMatLook fields and the nuclide densities), so that a lookup reads
+// its material from LDS instead of from memory.  te_off < 0: the material
+// takes material_sigt.
+struct DLdsMat {
+    int64_t key_base, te_off;
+    int32_t te_stride, n, nb, nuc_count;
+    double dens[kFastNuc];
+};
+constexpr int kLdsMats = 256;                    // materials the standalone lookup stages
 
 // The lookup kernels' Sigma_t: the LDS path when the material has one, else
 // material_sigt.

@@ -224,7 +224,11 @@ struct Engine::Impl {
     dim3 grid_dim() const { return dim3((unsigned)std::max(1, n_cu * cfg.grid_per_cu)); }
     // Launch shape of the standalone lookup kernel: with a bucket table in LDS
     // it takes most of a CU's LDS, so one workgroup of 1024 per CU.
-    size_t lookup_lds() const { return lds_grid >= 0 ? ((size_t)(lds_nb + 1) * 2 + 15) / 16 * 16 : 0; }
+    size_t lookup_lds() const
+    {
+        // the bucket table, then problem 0's materials
+        return lds_grid >= 0 ? ((size_t)(lds_nb + 1) * 2 + 15) / 16 * 16 + sizeof(DLdsMat) * kLdsMats : 0;
+    }
     dim3 lookup_grid() const { return lds_grid >= 0 ? dim3((unsigned)n_cu) : grid_dim(); }
     dim3 lookup_block() const { return lds_grid >= 0 ? dim3(1024) : dim3((unsigned)cfg.block); }
 };

@@ -2011,6 +2011,27 @@ __global__ void __launch_bounds__(1024) k_xs_lookup_array(KArgs A, const double*
                                                          double* __restrict__ out, uint64_t n)
 {
     const uint16_t* sb = stage_buckets(A, mc_lds_buckets);
+    // problem 0's materials (the lookup's) follow the bucket table in LDS
+    DLdsMat* lm = nullptr;
+    int nm = 0;
+    if (sb) {
+        lm = (DLdsMat*)((char*)mc_lds_buckets + ((size_t)(A.lds_nb + 1) * 2 + 15) / 16 * 16);
+        nm = A.prob[0].mat_count < kLdsMats ? A.prob[0].mat_count : kLdsMats;
+        for (int q = threadIdx.x; q < nm; q += blockDim.x) {
+            const DMatLook L = A.mlook[q];
+            DLdsMat R;
+            R.key_base = L.key_base;
+            R.te_off = L.te_off >= 0 && L.grid == A.lds_grid ? L.te_off : -1;
+            R.te_stride = L.te_stride;
+            R.n = L.n;
+            R.nb = L.nb;
+            R.nuc_count = L.nuc_count;
+#pragma unroll
+            for (int k = 0; k < kFastNuc; ++k) R.dens[k] = k < L.nuc_count ? A.mat_dens[L.nuc_begin + k] : 0.0;
+            lm[q] = R;
+        }
+        __syncthreads();
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += kU * stride) {
         int m[kU];
@@ -2024,7 +2045,14 @@ __global__ void __launch_bounds__(1024) k_xs_lookup_array(KArgs A, const double*
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             int g = 0;
-            r[u] = m[u] >= 0 ? lookup_sigt(A, sb, m[u], e[u], g) : 0.0;
+            if (m[u] < 0) {
+                r[u] = 0.0;
+            } else if (m[u] < nm && lm[m[u]].te_off >= 0) {
+                const DLdsMat& L = lm[m[u]];
+                r[u] = sigt_lds_core(A, L.key_base, L.te_off, L.te_stride, L.n, L.nb, L.nuc_count, L.dens, sb, e[u], g);
+            } else {
+                r[u] = lookup_sigt(A, sb, m[u], e[u], g);
+            }
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
