@@ -222,6 +222,12 @@ typedef struct mc_engine_cfg {
                                 collision, a particle takes per event iteration
                                 before the bank is refilled (0: 16; 1: one);
                                 runs that can split take one                  */
+    int32_t grad_events_per_launch; /* the same for gradient runs (0: 24; 1: one,
+                                with a collision kernel of its own)           */
+    int32_t grad_live_divisor; /* gradient runs keep at most
+                                bank_capacity / grad_live_divisor particles
+                                live (0: 4): the score-function rows of fewer
+                                slots stay in cache                           */
 } mc_engine_cfg;
 
 /* Per-kernel device time of the event loop, recorded when cfg.profile = 1:

@@ -35,7 +35,8 @@ class EngineCfg(C.Structure):
     _fields_ = [("device", C.c_int32), ("block", C.c_int32), ("grid_per_cu", C.c_int32),
                 ("iters_per_graph", C.c_int32), ("bank_capacity", C.c_uint64),
                 ("use_graph", C.c_int32), ("profile", C.c_int32), ("drain_divisor", C.c_int32),
-                ("events_per_launch", C.c_int32)]
+                ("events_per_launch", C.c_int32), ("grad_events_per_launch", C.c_int32),
+                ("grad_live_divisor", C.c_int32)]
 
 
 class TraceRecord(C.Structure):
@@ -247,13 +248,15 @@ class Engine:
 
     def __init__(self, problems, device: int = 0, block: int = 0, grid_per_cu: int = 0,
                  iters_per_graph: int = 0, bank_capacity: int = 0, use_graph: bool = True,
-                 profile: bool = False, drain_divisor: int = 0, events_per_launch: int = 0):
+                 profile: bool = False, drain_divisor: int = 0, events_per_launch: int = 0,
+                 grad_events_per_launch: int = 0, grad_live_divisor: int = 0):
         if isinstance(problems, Problem):
             problems = [problems]
         self.problems = list(problems)
         arr = (C.c_void_p * len(self.problems))(*[p._h.value for p in self.problems])
         cfg = EngineCfg(device, block, grid_per_cu, iters_per_graph, bank_capacity, int(use_graph), int(profile),
-                        int(drain_divisor), int(events_per_launch))
+                        int(drain_divisor), int(events_per_launch), int(grad_events_per_launch),
+                        int(grad_live_divisor))
         h = C.c_void_p()
         _check(lib.mc_engine_create(arr, len(self.problems), C.byref(cfg), C.byref(h)))
         self._h = h

@@ -216,6 +216,7 @@ struct KArgs {
     uint32_t events_per_launch;
     uint64_t* wg_xfl;
     int32_t n_prob;
+    int32_t n_mat;                // materials of all problems
     int32_t bin_stride;           // hist row length (max bins per problem)
     int32_t par_stride;           // rows of S and H (max params per problem)
     int32_t pad;
@@ -419,9 +420,24 @@ __device__ __forceinline__ const double* rec_at(const double* base, int64_t off,
 // two records the interval can use, all loaded at once, then the choice.  Same
 // interval (binary_search, Algorithm.cpp:46-64), same interpolation and the
 // same sum as material_sigt, so the same bits.  False: m has no fast path.
+__device__ __forceinline__ bool material_sigt_fast_at(const KArgs& A, const DMatLook& L, const double* dens,
+                                                     const double* ns, const double* nc, double E, double& sig,
+                                                     int& idx);
+
 __device__ __forceinline__ bool material_sigt_fast(const KArgs& A, int m, double E, double& sig, int& idx)
 {
     const DMatLook L = A.mlook[m];
+    const int j = L.nuc_begin > 0 ? L.nuc_begin : 0;
+    return material_sigt_fast_at(A, L, A.mat_dens + j, A.mat_ns + j, A.mat_nc + j, E, sig, idx);
+}
+
+// material_sigt_fast for the material of lookup record L; dens, ns and nc are
+// its nuclides' densities (unscaled materials) or per-reaction densities
+// (scaled ones).
+__device__ __forceinline__ bool material_sigt_fast_at(const KArgs& A, const DMatLook& L, const double* dens,
+                                                     const double* ns, const double* nc, double E, double& sig,
+                                                     int& idx)
+{
     if (L.rec_off < 0) return false;
     const int64_t key = (int64_t)(double_to_bits(E) >> kBucketShift) - L.key_base;
     int i, r0, r1;
@@ -473,7 +489,7 @@ __device__ __forceinline__ bool material_sigt_fast(const KArgs& A, int m, double
         double t = 0.0;
 #pragma unroll
         for (int k = 0; k < kFastNuc; ++k)
-            if (k < L.nuc_count) t += xs_pair(v + 2 * k, P) * A.mat_dens[L.nuc_begin + k];
+            if (k < L.nuc_count) t += xs_pair(v + 2 * k, P) * dens[k];
         sig = t;
         return true;
     }
@@ -481,9 +497,8 @@ __device__ __forceinline__ bool material_sigt_fast(const KArgs& A, int m, double
 #pragma unroll
     for (int k = 0; k < kFastNuc; ++k) {
         if (k < L.nuc_count) {
-            const int j = L.nuc_begin + k;
-            s += xs_pair(v + 2 * k, P) * A.mat_ns[j];
-            c += xs_pair(v + 2 * kFastNuc + 2 * k, P) * A.mat_nc[j];
+            s += xs_pair(v + 2 * k, P) * ns[k];
+            c += xs_pair(v + 2 * kFastNuc + 2 * k, P) * nc[k];
         }
     }
     sig = s + c;
@@ -594,6 +609,26 @@ __device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLo
 {
     return sigt_lds_core(A, L.key_base, L.te_off, L.te_stride, L.n, L.nb, L.nuc_count, A.mat_dens + L.nuc_begin, sb,
                          E, idx);
+}
+
+// The flight kernel's copy in LDS of the first kFlightLooks materials' lookup
+// records and nuclide densities (and per-reaction densities), which the
+// lookup after each collision reads.
+struct DFlightLook {
+    DMatLook L;
+    double dens[kFastNuc], ns[kFastNuc], nc[kFastNuc];   // per nuclide slot
+};
+constexpr int kFlightLooks = 32;
+
+// Sigma_t for the flight kernel's lookups: the material's record from LDS when
+// staged, else material_sigt (the same bits either way).
+__device__ __forceinline__ double flight_sigt(const KArgs& A, const DFlightLook* lk, int nlk, int m, double E, int& idx)
+{
+    if (m < nlk) {
+        double s;
+        if (material_sigt_fast_at(A, lk[m].L, lk[m].dens, lk[m].ns, lk[m].nc, E, s, idx)) return s;
+    }
+    return material_sigt(A, m, E, &idx);
 }
 
 // The standalone lookup's copy in LDS of what its path reads per material

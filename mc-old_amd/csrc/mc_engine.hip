@@ -843,6 +843,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.run_jd = d_run_jd.as<uint64_t>();
     A.hb_idx = d_hb_idx.as<int32_t>();
     A.n_prob = (int32_t)prob.size();
+    A.n_mat = (int32_t)mat.size();
     A.bin_stride = bin_stride;
     A.par_stride = par_stride;
     A.sstride = lr_stride(bank_cap);
@@ -867,7 +868,7 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.wg_xfl = d_wgxfl.as<uint64_t>();
     // a run that can split takes one flight per particle per iteration: its
     // copies wait in the ring for free slots, which appear once per iteration
-    A.events_per_launch = split_any ? 1u : (uint32_t)cfg.events_per_launch;
+    A.events_per_launch = split_any ? 1u : (uint32_t)(grad ? cfg.grad_events_per_launch : cfg.events_per_launch);
     A.hist = b_hist.as<double>();
     A.S = grad ? b_S.as<double>() : nullptr;
     A.H = grad ? b_H.as<double>() : nullptr;
@@ -1275,8 +1276,10 @@ void Engine::Impl::run(const mc_run_cfg* run, bool grad, const double* gout, mc_
         Counters c{};
         c.next_hist = plain ? hb[p] : 0;
         c.end_hist = plain ? hb[p + 1] : hb[np];
-        // slots worth scanning: no more than the histories of this run
-        c.active_cap = (uint32_t)std::min<uint64_t>(bank_cap, c.end_hist - c.next_hist);
+        // slots worth scanning: no more than the histories of this run (and,
+        // in gradient runs, than a share of the bank)
+        const uint64_t live = grad ? std::max<uint64_t>(1024, bank_cap / (uint64_t)cfg.grad_live_divisor) : bank_cap;
+        c.active_cap = (uint32_t)std::min<uint64_t>(std::min(live, bank_cap), c.end_hist - c.next_hist);
         h_active_cap = c.active_cap;
         HIP_CHECK(hipMemcpyAsync(A.ctr, &c, sizeof(Counters), hipMemcpyHostToDevice, cur));
         hipLaunchKernelGGL(k_fill_u8, fg, fb, 0, cur, A.status, bank_cap, (uint8_t)ST_EMPTY);
@@ -1396,6 +1399,11 @@ Engine::Engine(const std::vector<const Problem*>& ps, const mc_engine_cfg* c) : 
     if (cfg.events_per_launch == 0) cfg.events_per_launch = 16;
     if (cfg.events_per_launch < 1 || cfg.events_per_launch > 64)
         throw Error(MC_ERR_ARGUMENT, "events_per_launch must be in 1..64");
+    // gradient runs: measured best on the 24-parameter shield (1e8 histories)
+    if (cfg.grad_events_per_launch == 0) cfg.grad_events_per_launch = 24;
+    if (cfg.grad_live_divisor == 0) cfg.grad_live_divisor = 4;
+    if (cfg.grad_events_per_launch < 1 || cfg.grad_events_per_launch > 64 || cfg.grad_live_divisor < 1)
+        throw Error(MC_ERR_ARGUMENT, "grad_events_per_launch must be in 1..64, grad_live_divisor at least 1");
     if (cfg.block % 64 || cfg.block > 1024) throw Error(MC_ERR_ARGUMENT, "block must be a multiple of 64, <= 1024");
     if (cfg.bank_capacity > (1ull << 31)) throw Error(MC_ERR_CAPACITY, "bank capacity above 2^31");
     d->cfg = cfg;
@@ -1611,6 +1619,9 @@ void Engine::set_cfg(const mc_engine_cfg& c)
     d->cfg.use_graph = c.use_graph;
     d->cfg.drain_divisor = c.drain_divisor == 0 ? 4 : c.drain_divisor;
     d->cfg.events_per_launch = c.events_per_launch == 0 ? 16 : c.events_per_launch;
+    if (c.grad_events_per_launch >= 1 && c.grad_events_per_launch <= 64)
+        d->cfg.grad_events_per_launch = c.grad_events_per_launch;
+    if (c.grad_live_divisor >= 1) d->cfg.grad_live_divisor = c.grad_live_divisor;
     if (c.bank_capacity != d->cfg.bank_capacity) d->cfg.bank_capacity = c.bank_capacity;
     ++d->generation;
 }

@@ -1553,6 +1553,20 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     __shared__ uint32_t xq[kXq];
     __shared__ uint32_t xn, cn, sn;
     __shared__ unsigned long long xf;
+    // the lookup records of the first materials (the fused loop's lookups)
+    __shared__ DFlightLook lk[kFlightLooks];
+    const int nlk = kFuse && kTables ? (A.n_mat < kFlightLooks ? A.n_mat : kFlightLooks) : 0;
+    for (int q = threadIdx.x; q < nlk; q += blockDim.x) {
+        const DMatLook L = A.mlook[q];
+        lk[q].L = L;
+#pragma unroll
+        for (int k = 0; k < kFastNuc; ++k) {
+            const bool on = L.rec_off >= 0 && k < L.nuc_count;
+            lk[q].dens[k] = on ? A.mat_dens[L.nuc_begin + k] : 0.0;
+            lk[q].ns[k] = on ? A.mat_ns[L.nuc_begin + k] : 0.0;
+            lk[q].nc[k] = on ? A.mat_nc[L.nuc_begin + k] : 0.0;
+        }
+    }
     const uint32_t n = A.ctr->nA;
     uint32_t lo, hi;
     block_range(n, lo, hi);
@@ -1595,7 +1609,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                     if constexpr (kTables) {
                         // Sigma_t and its grid interval at the energy the collision left
                         int g = 0;
-                        R.sigt = m >= 0 ? lookup_sigt(A, nullptr, m, R.E, g) : 0.0;
+                        R.sigt = m >= 0 ? flight_sigt(A, lk, nlk, m, R.E, g) : 0.0;
                         R.gidx = g;
                     }
                 }

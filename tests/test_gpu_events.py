@@ -100,14 +100,15 @@ def test_splitting_with_several_events_per_launch():
     np.testing.assert_allclose(g.sq, o.sq, rtol=1e-12, atol=1e-300)
 
 
-@pytest.mark.parametrize("k", [1, 8])
-def test_gradient_any_events_per_launch(k):
+@pytest.mark.parametrize("k,live", [(1, 1), (8, 1), (24, 4)])
+def test_gradient_any_events_per_launch(k, live):
     """Gradient runs: at one flight per iteration the collision kernel takes
-    the collisions; at several the flight kernel folds them in the same pass.
-    The likelihood-ratio sums equal the oracle's either way."""
+    the collisions; at several the flight kernel folds them in the same pass;
+    with a quarter of the bank live (the default) or all of it.  The
+    likelihood-ratio sums equal the oracle's either way."""
     n = 4000
     p = mc.Problem.from_string(decks.shield_deck(n, params="24"), shield_xs_dir())
-    eng = mc.Engine(p, bank_capacity=1 << 12, events_per_launch=k)
+    eng = mc.Engine(p, bank_capacity=1 << 12, grad_events_per_launch=k, grad_live_divisor=live)
     rng = np.random.default_rng(4)
     go = rng.normal(size=p.n_bins)
     theta = rng.uniform(0.9, 1.1, p.n_params)
@@ -120,6 +121,7 @@ def test_gradient_any_events_per_launch(k):
 
 def test_events_per_launch_out_of_range():
     p = mc.Problem.from_string(decks.slab_deck(1.0, 0.5, 0, 100))
-    with pytest.raises(mc.MCError) as e:
-        mc.Engine(p, events_per_launch=65)
-    assert e.value.status == mc.MC_ERR_ARGUMENT
+    for kw in (dict(events_per_launch=65), dict(grad_events_per_launch=65), dict(grad_live_divisor=-1)):
+        with pytest.raises(mc.MCError) as e:
+            mc.Engine(p, **kw)
+        assert e.value.status == mc.MC_ERR_ARGUMENT
