@@ -217,6 +217,7 @@ struct KArgs {
     uint64_t* wg_xfl;
     int32_t n_prob;
     int32_t n_mat;                // materials of all problems
+    int32_t n_cell, n_surf, n_cell_surf;   // geometry of all problems
     int32_t bin_stride;           // hist row length (max bins per problem)
     int32_t par_stride;           // rows of S and H (max params per problem)
     int32_t pad;
@@ -614,6 +615,16 @@ __device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLo
 // The flight kernel's copy in LDS of the first kFlightLooks materials' lookup
 // records and nuclide densities (and per-reaction densities), which the
 // lookup after each collision reads.
+// The fused flight kernel's copy in LDS of the whole geometry (cells,
+// their surface lists, surfaces) when it is small enough, read by every
+// flight and collision instead of the engine's arrays.
+constexpr int kGeoCells = 64, kGeoSurfs = 64, kGeoCellSurf = 256;
+struct GeoLds {
+    mc_cell_desc cell[kGeoCells];
+    int32_t cell_surf[kGeoCellSurf];
+    mc_surface_desc surf[kGeoSurfs];
+};
+
 struct DFlightLook {
     DMatLook L;
     double dens[kFastNuc], ns[kFastNuc], nc[kFastNuc];   // per nuclide slot

@@ -26,7 +26,7 @@ template <bool kGrad> __global__ void k_refill_scatter(KArgs A);
 template <bool kGrad> __global__ void k_resume(KArgs A);
 __global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* c);
 __global__ void k_xs_lookup(KArgs A, int use_lds);
-template <bool kTables, bool kGrad, bool kFuse> __global__ void k_flight(KArgs A);
+template <bool kTables, bool kGrad, bool kFuse, bool kGeo> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
 template <bool kTables> __global__ void k_cross(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
@@ -844,6 +844,9 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.hb_idx = d_hb_idx.as<int32_t>();
     A.n_prob = (int32_t)prob.size();
     A.n_mat = (int32_t)mat.size();
+    A.n_cell = (int32_t)cell.size();
+    A.n_surf = (int32_t)surf.size();
+    A.n_cell_surf = (int32_t)cell_surf.size();
     A.bin_stride = bin_stride;
     A.par_stride = par_stride;
     A.sstride = lr_stride(bank_cap);
@@ -927,14 +930,20 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     // collide in the flight kernel
     const bool fused = !grad || A.events_per_launch > 1;
     const size_t acc_lds = sizeof(double) * kAccEntries * b.x;   // the fused gradient kernel's S sums
+    // fused runs whose geometry fits read it from an LDS copy
+    const bool geo = A.n_cell <= kGeoCells && A.n_surf <= kGeoSurfs && A.n_cell_surf <= kGeoCellSurf;
     if (has_tables) {
-        if (!grad) hipLaunchKernelGGL((k_flight<true, false, true>), g, b, 0, st, A);
-        else if (fused) hipLaunchKernelGGL((k_flight<true, true, true>), g, b, acc_lds, st, A);
-        else hipLaunchKernelGGL((k_flight<true, true, false>), g, b, 0, st, A);
+        if (!grad && geo) hipLaunchKernelGGL((k_flight<true, false, true, true>), g, b, 0, st, A);
+        else if (!grad) hipLaunchKernelGGL((k_flight<true, false, true, false>), g, b, 0, st, A);
+        else if (fused && geo) hipLaunchKernelGGL((k_flight<true, true, true, true>), g, b, acc_lds, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<true, true, true, false>), g, b, acc_lds, st, A);
+        else hipLaunchKernelGGL((k_flight<true, true, false, false>), g, b, 0, st, A);
     } else {
-        if (!grad) hipLaunchKernelGGL((k_flight<false, false, true>), g, b, 0, st, A);
-        else if (fused) hipLaunchKernelGGL((k_flight<false, true, true>), g, b, acc_lds, st, A);
-        else hipLaunchKernelGGL((k_flight<false, true, false>), g, b, 0, st, A);
+        if (!grad && geo) hipLaunchKernelGGL((k_flight<false, false, true, true>), g, b, 0, st, A);
+        else if (!grad) hipLaunchKernelGGL((k_flight<false, false, true, false>), g, b, 0, st, A);
+        else if (fused && geo) hipLaunchKernelGGL((k_flight<false, true, true, true>), g, b, acc_lds, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<false, true, true, false>), g, b, acc_lds, st, A);
+        else hipLaunchKernelGGL((k_flight<false, true, false, false>), g, b, 0, st, A);
     }
     // gradient runs cross surfaces in a kernel of their own (timed with the flight)
     if (grad) {

@@ -1422,19 +1422,21 @@ __device__ __forceinline__ void store_state(const KArgs& A, uint32_t s, const PS
     }
 }
 
-template <bool kTables, bool kGrad, bool kDefer, bool kReg = false>
-__device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& crossed, PState* R = nullptr)
+// kGeo: the geometry is read from the LDS copy *gl.
+template <bool kTables, bool kGrad, bool kDefer, bool kReg = false, bool kGeo = false>
+__device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& crossed, PState* R = nullptr,
+                                              const GeoLds* gl = nullptr)
 {
     const int c = kReg ? R->cell : A.pcell[s];
-    const mc_cell_desc C = A.cell[c];
+    const mc_cell_desc C = kGeo ? gl->cell[c] : A.cell[c];
     Vec3 p = kReg ? R->pos : Vec3{A.x[s], A.y[s], A.z[s]};
     const Vec3 u = kReg ? R->dir : Vec3{A.u[s], A.v[s], A.w[s]};
     // surface_intersect, general.cpp:54-67
     double dist = kMaxFloat;
     int S = -1;
     for (int k = 0; k < C.surf_count; ++k) {
-        const int sf = A.cell_surf[C.surf_begin + k];
-        const mc_surface_desc& Sd = A.surf[sf];
+        const int sf = kGeo ? gl->cell_surf[C.surf_begin + k] : A.cell_surf[C.surf_begin + k];
+        const mc_surface_desc& Sd = kGeo ? gl->surf[sf] : A.surf[sf];
         const double d = surface_distance(Sd.type, Sd.c, p, u);
         if (d < dist) { dist = d; S = sf; }
     }
@@ -1527,11 +1529,13 @@ __device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
 // once the queue fills or the range ends, so the rare crossing does not hold
 // a wave's other lanes and its registers stay out of the flight loop.
 struct PState;
-template <bool kTables, bool kGrad, bool kInlineFreeGas = true, bool kReg = false>
+template <bool kTables, bool kGrad, bool kInlineFreeGas = true, bool kReg = false, bool kGeo = false>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg = nullptr,
-                                                 double* sacc = nullptr);
+                                                 double* sacc = nullptr, const GeoLds* gl = nullptr);
 
-template <bool kTables, bool kGrad, bool kFuse>
+// kGeo (fused launches whose geometry fits GeoLds): flights and
+// collisions read the cells and surfaces from the workgroup's LDS copy.
+template <bool kTables, bool kGrad, bool kFuse, bool kGeo>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
     // kFuse: the flight's collision happens in the same pass (a kernel of its
@@ -1553,6 +1557,12 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     __shared__ uint32_t xq[kXq];
     __shared__ uint32_t xn, cn, sn;
     __shared__ unsigned long long xf;
+    __shared__ GeoLds geo;
+    if constexpr (kGeo) {
+        for (int q = threadIdx.x; q < A.n_cell; q += blockDim.x) geo.cell[q] = A.cell[q];
+        for (int q = threadIdx.x; q < A.n_surf; q += blockDim.x) geo.surf[q] = A.surf[q];
+        for (int q = threadIdx.x; q < A.n_cell_surf; q += blockDim.x) geo.cell_surf[q] = A.cell_surf[q];
+    }
     // the lookup records of the first materials (the fused loop's lookups)
     __shared__ DFlightLook lk[kFlightLooks];
     const int nlk = kFuse && kTables ? (A.n_mat < kFlightLooks ? A.n_mat : kFlightLooks) : 0;
@@ -1595,10 +1605,10 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                     // the slot's field addresses are recomputed where they are
                     // used (one add each) rather than held across the events
                     const uint32_t se = opaque(s);
-                    st = flight_one<kTables, kGrad, true, true>(A, se, crossed, &R);
+                    st = flight_one<kTables, kGrad, true, true, kGeo>(A, se, crossed, &R, &geo);
                     ++nfl;
                     if (st != ST_COLL) break;
-                    st = collision_one<kTables, kGrad, false, true>(A, se, &R, acc);
+                    st = collision_one<kTables, kGrad, false, true, kGeo>(A, se, &R, acc, &geo);
                     if (kGrad && st != ST_ALIVE) acc = nullptr;     // its end flushed the sums
                     if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
                     // once fewer than a quarter of the wave's lanes are still
@@ -1720,11 +1730,12 @@ __device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
 
 // kReg: the particle's state is *reg, read and updated there; the returned
 // status is not stored (the caller stores the state and the status).
-template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg>
-__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg, double* sacc)
+template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg, bool kGeo>
+__device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg, double* sacc,
+                                                 const GeoLds* gl)
 {
     const int c = kReg ? reg->cell : A.pcell[s];
-    const mc_cell_desc C = A.cell[c];
+    const mc_cell_desc C = kGeo ? gl->cell[c] : A.cell[c];
     const int p_id = kReg ? reg->prob : A.pprob[s];
     const DProb PR = A.prob[p_id];
     double w = kReg ? reg->w : A.wgt[s];
@@ -2219,12 +2230,16 @@ template __global__ void k_drain<false, true>(KArgs, uint32_t);
 template __global__ void k_drain<true, true>(KArgs, uint32_t);
 template __global__ void k_collision<false, true>(KArgs);
 template __global__ void k_collision<true, true>(KArgs);
-template __global__ void k_flight<false, false, true>(KArgs);
-template __global__ void k_flight<true, false, true>(KArgs);
-template __global__ void k_flight<false, true, false>(KArgs);
-template __global__ void k_flight<true, true, false>(KArgs);
-template __global__ void k_flight<false, true, true>(KArgs);
-template __global__ void k_flight<true, true, true>(KArgs);
+template __global__ void k_flight<false, false, true, false>(KArgs);
+template __global__ void k_flight<false, false, true, true>(KArgs);
+template __global__ void k_flight<true, false, true, false>(KArgs);
+template __global__ void k_flight<true, false, true, true>(KArgs);
+template __global__ void k_flight<false, true, false, false>(KArgs);
+template __global__ void k_flight<true, true, false, false>(KArgs);
+template __global__ void k_flight<false, true, true, false>(KArgs);
+template __global__ void k_flight<false, true, true, true>(KArgs);
+template __global__ void k_flight<true, true, true, false>(KArgs);
+template __global__ void k_flight<true, true, true, true>(KArgs);
 template __global__ void k_cross<false>(KArgs);
 template __global__ void k_cross<true>(KArgs);
 
