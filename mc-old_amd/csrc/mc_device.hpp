@@ -337,17 +337,23 @@ __device__ __forceinline__ void xs_point(const KArgs& A, int gid, double E, XsPo
 }
 
 // The same point from an interval index found earlier at this E.
-__device__ __forceinline__ void xs_point_at(const KArgs& A, int gid, int idx, double E, XsPoint& P)
+__device__ __forceinline__ void xs_point_on(const KArgs& A, int gid, int e_off, int n, int idx, double E,
+                                            XsPoint& P)
 {
-    const DGrid g = A.grid[gid];
     P.grid = gid;
-    P.n = g.n;
+    P.n = n;
     P.idx = idx;
-    if (idx >= 0 && idx < g.n - 1) {
-        const double E1 = A.grid_E[g.e_off + idx], E2 = A.grid_E[g.e_off + idx + 1];
+    if (idx >= 0 && idx < n - 1) {
+        const double E1 = A.grid_E[e_off + idx], E2 = A.grid_E[e_off + idx + 1];
         P.a = (E - E2) / (E1 - E2);
         P.b = (E - E1) / (E2 - E1);
     }
+}
+
+__device__ __forceinline__ void xs_point_at(const KArgs& A, int gid, int idx, double E, XsPoint& P)
+{
+    const DGrid g = A.grid[gid];
+    xs_point_on(A, gid, g.e_off, g.n, idx, E, P);
 }
 
 // XSTable::xs (XSec.cpp:9-40) with the interpolation of Algorithm.cpp:103-105
@@ -625,9 +631,14 @@ struct GeoLds {
     mc_surface_desc surf[kGeoSurfs];
 };
 
+// The lookup record also carries what the material's collisions read: its
+// cross-section records and, per nuclide slot, the target mass and kinematics.
 struct DFlightLook {
     DMatLook L;
+    DMatRec R;
     double dens[kFastNuc], ns[kFastNuc], nc[kFastNuc];   // per nuclide slot
+    double mass[kFastNuc];
+    int32_t kin[kFastNuc];
 };
 constexpr int kFlightLooks = 32;
 

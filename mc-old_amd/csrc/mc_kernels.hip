@@ -1387,6 +1387,7 @@ struct PState {
     uint64_t rng;
     int cell, prob, gidx;
     uint32_t nfl;
+    bool pend;                    // gradient runs: the slot's cs, cc or Qw may be non-zero
 };
 
 __device__ __forceinline__ PState load_state(const KArgs& A, uint32_t s, bool tables)
@@ -1402,6 +1403,7 @@ __device__ __forceinline__ PState load_state(const KArgs& A, uint32_t s, bool ta
     P.nfl = A.nflight[s];
     P.sigt = tables ? A.sigt[s] : 0.0;
     P.lp = 0.0;                   // every launch's first event is a flight
+    P.pend = true;
     P.gidx = tables ? A.gidx[s] : 0;
     return P;
 }
@@ -1470,6 +1472,7 @@ __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& 
     const double w = kReg ? R->w : A.wgt[s];
     for (int a = 0; a < C.tl_count; ++a)
         score_attachment<kGrad>(A, s, C.tl_begin + a, l, c, E, w, PR, l, sigt, !hit);
+    if (kGrad && kReg && C.tl_count > 0) R->pend = true;
     uint8_t st;
     crossed = false;
     if (kReg) {
@@ -1531,7 +1534,8 @@ __device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
 struct PState;
 template <bool kTables, bool kGrad, bool kInlineFreeGas = true, bool kReg = false, bool kGeo = false>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg = nullptr,
-                                                 double* sacc = nullptr, const GeoLds* gl = nullptr);
+                                                 double* sacc = nullptr, const GeoLds* gl = nullptr,
+                                                 const DFlightLook* lk = nullptr, int nlk = 0);
 
 // kGeo (fused launches whose geometry fits GeoLds): flights and
 // collisions read the cells and surfaces from the workgroup's LDS copy.
@@ -1575,7 +1579,10 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
             lk[q].dens[k] = on ? A.mat_dens[L.nuc_begin + k] : 0.0;
             lk[q].ns[k] = on ? A.mat_ns[L.nuc_begin + k] : 0.0;
             lk[q].nc[k] = on ? A.mat_nc[L.nuc_begin + k] : 0.0;
+            lk[q].mass[k] = on ? A.nuc[A.mat_nuc[L.nuc_begin + k]].A : 0.0;
+            lk[q].kin[k] = on ? A.nuc[A.mat_nuc[L.nuc_begin + k]].kin : 0;
         }
+        lk[q].R = A.mrec[q];
     }
     const uint32_t n = A.ctr->nA;
     uint32_t lo, hi;
@@ -1595,6 +1602,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
             if constexpr (kFuse) {
                 // the state stays in registers from event to event
                 PState R = load_state(A, s, kTables);
+                if constexpr (kGrad) R.pend = A.Qw[s] != 0.0 || A.cs[s] != 0.0 || A.cc[s] != 0.0;
                 const int m = kTables ? A.pmat[s] : -1;
                 // gradient runs: this lane's S sums in LDS, when its material allows
                 double* acc = nullptr;
@@ -1608,7 +1616,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
                     st = flight_one<kTables, kGrad, true, true, kGeo>(A, se, crossed, &R, &geo);
                     ++nfl;
                     if (st != ST_COLL) break;
-                    st = collision_one<kTables, kGrad, false, true, kGeo>(A, se, &R, acc, &geo);
+                    st = collision_one<kTables, kGrad, false, true, kGeo>(A, se, &R, acc, &geo, lk, nlk);
                     if (kGrad && st != ST_ALIVE) acc = nullptr;     // its end flushed the sums
                     if (st != ST_ALIVE || nfl >= A.events_per_launch) break;
                     // once fewer than a quarter of the wave's lanes are still
@@ -1730,9 +1738,11 @@ __device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
 
 // kReg: the particle's state is *reg, read and updated there; the returned
 // status is not stored (the caller stores the state and the status).
+// lk, nlk (fused launches): a material with a staged lookup record takes its
+// descriptor, grid, records, densities and nuclide masses from LDS.
 template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg, bool kGeo>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg, double* sacc,
-                                                 const GeoLds* gl)
+                                                 const GeoLds* gl, const DFlightLook* lk, int nlk)
 {
     const int c = kReg ? reg->cell : A.pcell[s];
     const mc_cell_desc C = kGeo ? gl->cell[c] : A.cell[c];
@@ -1751,15 +1761,26 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
             const double lp = kGrad ? (kReg ? reg->lp : A.lpend[s]) : 0.0;
             for (int a = 0; a < C.col_count; ++a)
                 score_attachment<kGrad>(A, s, C.col_begin + a, 0.0, c, E, w, PR, lp, 0.0, false);
+            if (kGrad && kReg) reg->pend = true;
         }
         const int m = C.material;
-        const mc_material_desc M = A.mat[m];
+        // (gradient runs keep their own reads: the pointer's registers cost
+        // them more than the reads)
+        const DFlightLook* F = !kGrad && kTables && kReg && m < nlk && lk[m].L.rec_off >= 0 ? &lk[m] : nullptr;
+        mc_material_desc M;
+        if (F && !kGrad) M = {F->L.nuc_begin, F->L.nuc_count, 0, F->L.scaled};
+        else M = A.mat[m];
         const bool scaled = M.param_count != 0;
         XsPoint P;
         int gm;
         if constexpr (kTables && kReg) {
-            gm = A.mat_grid[m];
-            if (gm >= 0) xs_point_at(A, gm, reg->gidx, E, P);
+            if (F) {
+                gm = F->L.grid;
+                xs_point_on(A, gm, F->L.e_off, F->L.n, reg->gidx, E, P);
+            } else {
+                gm = A.mat_grid[m];
+                if (gm >= 0) xs_point_at(A, gm, reg->gidx, E, P);
+            }
         } else {
             gm = kTables ? material_point_cached(A, m, s, E, P) : material_point(A, m, E, P);
         }
@@ -1774,7 +1795,7 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
         Xs3 X = {0.0, 0.0, 0.0};
         // sigma_s and sigma_c of every nuclide from the material's record (one
         // or two cache lines) when it has one, else from the nuclide tables
-        const DMatRec R = A.mrec[m];
+        const DMatRec R = F ? F->R : A.mrec[m];
         const double* rsc = kTables && R.sc_off >= 0 ? rec_at(A.rec_sc, R.sc_off, R.sc_stride, P) : nullptr;
         const int n2 = R.c_at;
 #pragma unroll
@@ -1782,17 +1803,18 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
             if (!kGrad) ps[k] = 0.0;
             if (k < M.nuc_count) {
                 const int j = M.nuc_begin + k;
-                const double Nd = A.mat_dens[j];
+                const bool f = F && k < kFastNuc;
+                const double Nd = f ? F->dens[k] : A.mat_dens[j];
                 const double xs_s = rsc ? xs_pair(rsc + 2 * k, P) : entry_xs(A, j, XS_S, E, P, gm);
                 const double xs_c = rsc ? xs_pair(rsc + n2 + 2 * k, P) : entry_xs(A, j, XS_C, E, P, gm);
-                const double prod = xs_s * (scaled ? A.mat_ns[j] : Nd);
+                const double prod = xs_s * (scaled ? (f ? F->ns[k] : A.mat_ns[j]) : Nd);
                 if constexpr (kGrad) {
                     rs[k] = xs_s;
                 } else {
                     ps[k] = prod;
                 }
                 X.s += prod;
-                X.c += xs_c * (scaled ? A.mat_nc[j] : Nd);
+                X.c += xs_c * (scaled ? (f ? F->nc[k] : A.mat_nc[j]) : Nd);
                 if (!kTables && !scaled) X.t += entry_xs(A, j, XS_T, E, P, gm) * Nd;
             }
         }
@@ -1842,7 +1864,8 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
             // read again (the record or table lines are in cache).
             if (M.param_count > 0 && A.mfold[m]) {
                 // per nuclide: its scatter and capture parameters, sigma_s from registers
-                LrPending L = lr_pending_cs(A, s);
+                // (a particle that has not scored has nothing pending)
+                LrPending L = kReg && !reg->pend ? LrPending{0.0, 0.0, 0.0, 0.0} : lr_pending_cs(A, s);
                 L.lp = kReg ? reg->lp : A.lpend[s];
 #pragma unroll
                 for (int k = 0; k < kNucReg; ++k) {
@@ -1871,7 +1894,8 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
                     A.cc[s] = 0.0;
                 }
             } else if (M.param_count > 0) {
-                LrPending L = lr_pending_cs(A, s);
+                // (a particle that has not scored has nothing pending)
+                LrPending L = kReg && !reg->pend ? LrPending{0.0, 0.0, 0.0, 0.0} : lr_pending_cs(A, s);
                 L.lp = kReg ? reg->lp : A.lpend[s];
                 for (int q = 0; q < M.param_count; ++q) {
                     const DParFold F = A.mpar[M.param_begin + q];
@@ -1904,10 +1928,20 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
                 }
             }
         }
-        const DNuc N = A.nuc[A.mat_nuc[M.nuc_begin + (sel >= 0 ? sel : 0)]];
-        if (sel >= 0 && N.kin == MC_KIN_FREE_GAS && N.A < MC_MASS_INFINITE) {
-            const FreeGasState f = kInlineFreeGas ? free_gas_inline(N.A, FreeGasState{E, u, rng})
-                                                  : free_gas_outline(N.A, FreeGasState{E, u, rng});
+        const int ss = sel >= 0 ? sel : 0;
+        double n_mass;
+        int n_kin;
+        if (F) {
+            n_mass = F->mass[ss];
+            n_kin = F->kin[ss];
+        } else {
+            const DNuc N = A.nuc[A.mat_nuc[M.nuc_begin + ss]];
+            n_mass = N.A;
+            n_kin = N.kin;
+        }
+        if (sel >= 0 && n_kin == MC_KIN_FREE_GAS && n_mass < MC_MASS_INFINITE) {
+            const FreeGasState f = kInlineFreeGas ? free_gas_inline(n_mass, FreeGasState{E, u, rng})
+                                                  : free_gas_outline(n_mass, FreeGasState{E, u, rng});
             E = f.E;
             u = f.u;
             rng = f.rng;
@@ -1915,8 +1949,8 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
             // elastic scatter, target at rest (T -> 0 limit of Reaction.cpp:27-118)
             const double mu_cm = 2.0 * urand(rng) - 1.0;
             double mu = mu_cm;
-            if (N.A < MC_MASS_INFINITE) {
-                const double Am = N.A;
+            if (n_mass < MC_MASS_INFINITE) {
+                const double Am = n_mass;
                 const double q = Am * Am + 2.0 * Am * mu_cm + 1.0;
                 E = E * q / ((Am + 1.0) * (Am + 1.0));
                 mu = (1.0 + Am * mu_cm) / sqrt(q);
