@@ -1764,9 +1764,7 @@ __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PSt
             if (kGrad && kReg) reg->pend = true;
         }
         const int m = C.material;
-        // (gradient runs keep their own reads: the pointer's registers cost
-        // them more than the reads)
-        const DFlightLook* F = !kGrad && kTables && kReg && m < nlk && lk[m].L.rec_off >= 0 ? &lk[m] : nullptr;
+        const DFlightLook* F = kTables && kReg && m < nlk && lk[m].L.rec_off >= 0 ? &lk[m] : nullptr;
         mc_material_desc M;
         if (F && !kGrad) M = {F->L.nuc_begin, F->L.nuc_count, 0, F->L.scaled};
         else M = A.mat[m];
