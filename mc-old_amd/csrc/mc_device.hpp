@@ -218,6 +218,8 @@ struct KArgs {
     int32_t n_prob;
     int32_t n_mat;                // materials of all problems
     int32_t n_cell, n_surf, n_cell_surf;   // geometry of all problems
+    const char* crec;             // per-cell flight records (DCellHead, DCellSurf...), or null
+    int32_t crec_stride;          // bytes from one cell's record to the next
     int32_t bin_stride;           // hist row length (max bins per problem)
     int32_t par_stride;           // rows of S and H (max params per problem)
     int32_t pad;
@@ -621,6 +623,23 @@ __device__ __forceinline__ double material_sigt_lds(const KArgs& A, const DMatLo
 // The flight kernel's copy in LDS of the first kFlightLooks materials' lookup
 // records and nuclide densities (and per-reaction densities), which the
 // lookup after each collision reads.
+// Per-cell flight records, a fixed stride apart (problem sets whose geometry
+// does not fit the LDS copy below): the cell fields a flight and a collision
+// read, then its surfaces' ids, types and coefficients in the cell's order,
+// so that a flight reads its geometry in one place instead of through the
+// cell, its surface list and the surfaces.
+struct DCellHead {
+    int32_t material, surf_count, tl_begin, tl_count, col_begin, col_count, pad0, pad1;
+};
+struct DCellSurf {
+    int32_t sid, type;
+    double c[5];                  // mc_surface_desc c[0..4]: all surface_distance reads
+};
+constexpr int kRecSurfMax = 8;    // cells of at most this many surfaces
+
+// Where the fused flight kernel reads the geometry from.
+enum { kGeoArrays = 0, kGeoLds = 1, kGeoRec = 2 };
+
 // The fused flight kernel's copy in LDS of the whole geometry (cells,
 // their surface lists, surfaces) when it is small enough, read by every
 // flight and collision instead of the engine's arrays.

@@ -26,7 +26,7 @@ template <bool kGrad> __global__ void k_refill_scatter(KArgs A);
 template <bool kGrad> __global__ void k_resume(KArgs A);
 __global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* c);
 __global__ void k_xs_lookup(KArgs A, int use_lds);
-template <bool kTables, bool kGrad, bool kFuse, bool kGeo> __global__ void k_flight(KArgs A);
+template <bool kTables, bool kGrad, bool kFuse, int kGeo> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
 template <bool kTables> __global__ void k_cross(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
@@ -132,6 +132,8 @@ struct Engine::Impl {
     std::vector<mc_surface_desc> surf;
     std::vector<mc_cell_desc> cell;
     std::vector<int32_t> cell_surf, cell_sense, nbr_begin, nbr;
+    std::vector<double> crec;          // per-cell flight records (as doubles: 8-byte aligned)
+    int32_t crec_stride = 0;
     std::vector<mc_material_desc> mat;
     std::vector<int32_t> mat_nuc, mat_param, mat_grid, me_tab;
     std::vector<double> me_cst;
@@ -165,6 +167,7 @@ struct Engine::Impl {
     double emin = 0, emax = 0;
 
     // device data
+    DevBuf d_crec;
     DevBuf d_surf, d_cell, d_cell_surf, d_cell_sense, d_nbr_begin, d_nbr, d_mat, d_mat_nuc, d_mat_dens,
         d_mat_ns, d_mat_nc, d_mat_tconst, d_mat_param, d_mat_grid, d_me_tab, d_me_cst, d_mrec, d_mlook, d_rec_te, d_bucket16, d_rec_t, d_rec_sc, d_param, d_theta, d_mpar, d_nfold, d_mfold, d_nuc, d_grid,
         d_grid_E, d_bucket, d_tab_s,
@@ -548,6 +551,25 @@ void Engine::Impl::build(const std::vector<const Problem*>& ps)
         nbr.insert(nbr.end(), L.begin(), L.end());
         nbr_begin.push_back((int)nbr.size());
     }
+    // per-cell flight records when every cell has at most kRecSurfMax surfaces
+    int smax = 0;
+    for (const mc_cell_desc& C : cell) smax = std::max(smax, C.surf_count);
+    if (!cell.empty() && smax <= kRecSurfMax) {
+        crec_stride = (int32_t)(sizeof(DCellHead) + sizeof(DCellSurf) * std::max(smax, 1));
+        crec.assign(cell.size() * (size_t)crec_stride / sizeof(double), 0.0);
+        for (size_t c = 0; c < cell.size(); ++c) {
+            char* r = (char*)crec.data() + c * (size_t)crec_stride;
+            const mc_cell_desc& C = cell[c];
+            const DCellHead H{C.material, C.surf_count, C.tl_begin, C.tl_count, C.col_begin, C.col_count, 0, 0};
+            std::memcpy(r, &H, sizeof H);
+            for (int k = 0; k < C.surf_count; ++k) {
+                const int sf = cell_surf[C.surf_begin + k];
+                DCellSurf S{sf, surf[sf].type, {}};
+                for (int q = 0; q < 5; ++q) S.c[q] = surf[sf].c[q];
+                std::memcpy(r + sizeof(DCellHead) + k * sizeof(DCellSurf), &S, sizeof S);
+            }
+        }
+    }
     // LCG jump multipliers: jump[256 j + v] = g^(stride * v * 256^j), one per
     // byte value of a history key
     jump.assign(8 * 256, 1);
@@ -564,6 +586,7 @@ void Engine::Impl::upload_static()
     upload(d_cell, cell, stream);
     upload(d_cell_surf, cell_surf, stream);
     upload(d_cell_sense, cell_sense, stream);
+    upload(d_crec, crec, stream);
     upload(d_nbr_begin, nbr_begin, stream);
     upload(d_nbr, nbr, stream);
     upload(d_mat, mat, stream);
@@ -766,6 +789,8 @@ KArgs Engine::Impl::args(bool grad, mc_trace_record* dtrace, uint64_t trace_coun
     A.surf = d_surf.as<mc_surface_desc>();
     A.cell = d_cell.as<mc_cell_desc>();
     A.cell_surf = d_cell_surf.as<int32_t>();
+    A.crec = crec.empty() ? nullptr : d_crec.as<char>();
+    A.crec_stride = crec_stride;
     A.cell_sense = d_cell_sense.as<int32_t>();
     A.nbr_begin = d_nbr_begin.as<int32_t>();
     A.nbr = d_nbr.as<int32_t>();
@@ -930,20 +955,26 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     // collide in the flight kernel
     const bool fused = !grad || A.events_per_launch > 1;
     const size_t acc_lds = sizeof(double) * kAccEntries * b.x;   // the fused gradient kernel's S sums
-    // fused runs whose geometry fits read it from an LDS copy
-    const bool geo = A.n_cell <= kGeoCells && A.n_surf <= kGeoSurfs && A.n_cell_surf <= kGeoCellSurf;
+    // fused runs read the geometry from an LDS copy when it fits; forward
+    // runs otherwise from the per-cell flight records when there are (in the
+    // gradient kernel the record's pointer costs more registers than the
+    // reads it saves)
+    const bool lds_geo = A.n_cell <= kGeoCells && A.n_surf <= kGeoSurfs && A.n_cell_surf <= kGeoCellSurf;
+    const bool rec_geo = !lds_geo && !grad && A.crec;
     if (has_tables) {
-        if (!grad && geo) hipLaunchKernelGGL((k_flight<true, false, true, true>), g, b, 0, st, A);
-        else if (!grad) hipLaunchKernelGGL((k_flight<true, false, true, false>), g, b, 0, st, A);
-        else if (fused && geo) hipLaunchKernelGGL((k_flight<true, true, true, true>), g, b, acc_lds, st, A);
-        else if (fused) hipLaunchKernelGGL((k_flight<true, true, true, false>), g, b, acc_lds, st, A);
-        else hipLaunchKernelGGL((k_flight<true, true, false, false>), g, b, 0, st, A);
+        if (!grad && lds_geo) hipLaunchKernelGGL((k_flight<true, false, true, kGeoLds>), g, b, 0, st, A);
+        else if (rec_geo) hipLaunchKernelGGL((k_flight<true, false, true, kGeoRec>), g, b, 0, st, A);
+        else if (!grad) hipLaunchKernelGGL((k_flight<true, false, true, kGeoArrays>), g, b, 0, st, A);
+        else if (fused && lds_geo) hipLaunchKernelGGL((k_flight<true, true, true, kGeoLds>), g, b, acc_lds, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<true, true, true, kGeoArrays>), g, b, acc_lds, st, A);
+        else hipLaunchKernelGGL((k_flight<true, true, false, kGeoArrays>), g, b, 0, st, A);
     } else {
-        if (!grad && geo) hipLaunchKernelGGL((k_flight<false, false, true, true>), g, b, 0, st, A);
-        else if (!grad) hipLaunchKernelGGL((k_flight<false, false, true, false>), g, b, 0, st, A);
-        else if (fused && geo) hipLaunchKernelGGL((k_flight<false, true, true, true>), g, b, acc_lds, st, A);
-        else if (fused) hipLaunchKernelGGL((k_flight<false, true, true, false>), g, b, acc_lds, st, A);
-        else hipLaunchKernelGGL((k_flight<false, true, false, false>), g, b, 0, st, A);
+        if (!grad && lds_geo) hipLaunchKernelGGL((k_flight<false, false, true, kGeoLds>), g, b, 0, st, A);
+        else if (rec_geo) hipLaunchKernelGGL((k_flight<false, false, true, kGeoRec>), g, b, 0, st, A);
+        else if (!grad) hipLaunchKernelGGL((k_flight<false, false, true, kGeoArrays>), g, b, 0, st, A);
+        else if (fused && lds_geo) hipLaunchKernelGGL((k_flight<false, true, true, kGeoLds>), g, b, acc_lds, st, A);
+        else if (fused) hipLaunchKernelGGL((k_flight<false, true, true, kGeoArrays>), g, b, acc_lds, st, A);
+        else hipLaunchKernelGGL((k_flight<false, true, false, kGeoArrays>), g, b, 0, st, A);
     }
     // gradient runs cross surfaces in a kernel of their own (timed with the flight)
     if (grad) {

@@ -1424,23 +1424,41 @@ __device__ __forceinline__ void store_state(const KArgs& A, uint32_t s, const PS
     }
 }
 
-// kGeo: the geometry is read from the LDS copy *gl.
-template <bool kTables, bool kGrad, bool kDefer, bool kReg = false, bool kGeo = false>
+// kGeo: where the geometry is read from (kGeoLds: the LDS copy *gl;
+// kGeoRec: the cell's flight record; kGeoArrays: cell, cell_surf, surf).
+template <bool kTables, bool kGrad, bool kDefer, bool kReg = false, int kGeo = kGeoArrays>
 __device__ __forceinline__ uint8_t flight_one(const KArgs& A, uint32_t s, bool& crossed, PState* R = nullptr,
                                               const GeoLds* gl = nullptr)
 {
     const int c = kReg ? R->cell : A.pcell[s];
-    const mc_cell_desc C = kGeo ? gl->cell[c] : A.cell[c];
+    mc_cell_desc C{};
+    const DCellSurf* csr = nullptr;
+    if constexpr (kGeo == kGeoRec) {
+        const char* rec = A.crec + (size_t)c * A.crec_stride;
+        const DCellHead H = *(const DCellHead*)rec;
+        C.material = H.material;
+        C.surf_count = H.surf_count;
+        C.tl_begin = H.tl_begin;
+        C.tl_count = H.tl_count;
+        csr = (const DCellSurf*)(rec + sizeof(DCellHead));
+    } else {
+        C = kGeo == kGeoLds ? gl->cell[c] : A.cell[c];
+    }
     Vec3 p = kReg ? R->pos : Vec3{A.x[s], A.y[s], A.z[s]};
     const Vec3 u = kReg ? R->dir : Vec3{A.u[s], A.v[s], A.w[s]};
     // surface_intersect, general.cpp:54-67
     double dist = kMaxFloat;
     int S = -1;
     for (int k = 0; k < C.surf_count; ++k) {
-        const int sf = kGeo ? gl->cell_surf[C.surf_begin + k] : A.cell_surf[C.surf_begin + k];
-        const mc_surface_desc& Sd = kGeo ? gl->surf[sf] : A.surf[sf];
-        const double d = surface_distance(Sd.type, Sd.c, p, u);
-        if (d < dist) { dist = d; S = sf; }
+        if constexpr (kGeo == kGeoRec) {
+            const double d = surface_distance(csr[k].type, csr[k].c, p, u);
+            if (d < dist) { dist = d; S = csr[k].sid; }
+        } else {
+            const int sf = kGeo == kGeoLds ? gl->cell_surf[C.surf_begin + k] : A.cell_surf[C.surf_begin + k];
+            const mc_surface_desc& Sd = kGeo == kGeoLds ? gl->surf[sf] : A.surf[sf];
+            const double d = surface_distance(Sd.type, Sd.c, p, u);
+            if (d < dist) { dist = d; S = sf; }
+        }
     }
     // collision_distance, general.cpp:40-48
     uint64_t rng = kReg ? R->rng : A.rng[s];
@@ -1532,14 +1550,15 @@ __device__ __forceinline__ void cross_deferred(const KArgs& A, uint32_t s)
 // once the queue fills or the range ends, so the rare crossing does not hold
 // a wave's other lanes and its registers stay out of the flight loop.
 struct PState;
-template <bool kTables, bool kGrad, bool kInlineFreeGas = true, bool kReg = false, bool kGeo = false>
+template <bool kTables, bool kGrad, bool kInlineFreeGas = true, bool kReg = false, int kGeo = kGeoArrays>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg = nullptr,
                                                  double* sacc = nullptr, const GeoLds* gl = nullptr,
                                                  const DFlightLook* lk = nullptr, int nlk = 0);
 
-// kGeo (fused launches whose geometry fits GeoLds): flights and
-// collisions read the cells and surfaces from the workgroup's LDS copy.
-template <bool kTables, bool kGrad, bool kFuse, bool kGeo>
+// kGeo: kGeoLds (fused launches whose geometry fits GeoLds): flights and
+// collisions read the cells and surfaces from the workgroup's LDS copy;
+// kGeoRec: from the per-cell flight records.
+template <bool kTables, bool kGrad, bool kFuse, int kGeo>
 __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 {
     // kFuse: the flight's collision happens in the same pass (a kernel of its
@@ -1562,7 +1581,7 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     __shared__ uint32_t xn, cn, sn;
     __shared__ unsigned long long xf;
     __shared__ GeoLds geo;
-    if constexpr (kGeo) {
+    if constexpr (kGeo == kGeoLds) {
         for (int q = threadIdx.x; q < A.n_cell; q += blockDim.x) geo.cell[q] = A.cell[q];
         for (int q = threadIdx.x; q < A.n_surf; q += blockDim.x) geo.surf[q] = A.surf[q];
         for (int q = threadIdx.x; q < A.n_cell_surf; q += blockDim.x) geo.cell_surf[q] = A.cell_surf[q];
@@ -1740,12 +1759,20 @@ __device__ __noinline__ FreeGasState free_gas_outline(double A, FreeGasState f)
 // status is not stored (the caller stores the state and the status).
 // lk, nlk (fused launches): a material with a staged lookup record takes its
 // descriptor, grid, records, densities and nuclide masses from LDS.
-template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg, bool kGeo>
+template <bool kTables, bool kGrad, bool kInlineFreeGas, bool kReg, int kGeo>
 __device__ __forceinline__ uint8_t collision_one(const KArgs& A, uint32_t s, PState* reg, double* sacc,
                                                  const GeoLds* gl, const DFlightLook* lk, int nlk)
 {
     const int c = kReg ? reg->cell : A.pcell[s];
-    const mc_cell_desc C = kGeo ? gl->cell[c] : A.cell[c];
+    mc_cell_desc C{};
+    if constexpr (kGeo == kGeoRec) {
+        const DCellHead H = *(const DCellHead*)(A.crec + (size_t)c * A.crec_stride);
+        C.material = H.material;
+        C.col_begin = H.col_begin;
+        C.col_count = H.col_count;
+    } else {
+        C = kGeo == kGeoLds ? gl->cell[c] : A.cell[c];
+    }
     const int p_id = kReg ? reg->prob : A.pprob[s];
     const DProb PR = A.prob[p_id];
     double w = kReg ? reg->w : A.wgt[s];
@@ -2262,16 +2289,18 @@ template __global__ void k_drain<false, true>(KArgs, uint32_t);
 template __global__ void k_drain<true, true>(KArgs, uint32_t);
 template __global__ void k_collision<false, true>(KArgs);
 template __global__ void k_collision<true, true>(KArgs);
-template __global__ void k_flight<false, false, true, false>(KArgs);
-template __global__ void k_flight<false, false, true, true>(KArgs);
-template __global__ void k_flight<true, false, true, false>(KArgs);
-template __global__ void k_flight<true, false, true, true>(KArgs);
-template __global__ void k_flight<false, true, false, false>(KArgs);
-template __global__ void k_flight<true, true, false, false>(KArgs);
-template __global__ void k_flight<false, true, true, false>(KArgs);
-template __global__ void k_flight<false, true, true, true>(KArgs);
-template __global__ void k_flight<true, true, true, false>(KArgs);
-template __global__ void k_flight<true, true, true, true>(KArgs);
+template __global__ void k_flight<false, false, true, kGeoArrays>(KArgs);
+template __global__ void k_flight<false, true, true, kGeoArrays>(KArgs);
+template __global__ void k_flight<false, false, true, kGeoLds>(KArgs);
+template __global__ void k_flight<false, true, true, kGeoLds>(KArgs);
+template __global__ void k_flight<false, false, true, kGeoRec>(KArgs);
+template __global__ void k_flight<false, true, false, kGeoArrays>(KArgs);
+template __global__ void k_flight<true, false, true, kGeoArrays>(KArgs);
+template __global__ void k_flight<true, true, true, kGeoArrays>(KArgs);
+template __global__ void k_flight<true, false, true, kGeoLds>(KArgs);
+template __global__ void k_flight<true, true, true, kGeoLds>(KArgs);
+template __global__ void k_flight<true, false, true, kGeoRec>(KArgs);
+template __global__ void k_flight<true, true, false, kGeoArrays>(KArgs);
 template __global__ void k_cross<false>(KArgs);
 template __global__ void k_cross<true>(KArgs);
 
