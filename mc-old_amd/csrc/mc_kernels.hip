@@ -1606,6 +1606,11 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     const uint32_t n = A.ctr->nA;
     uint32_t lo, hi;
     block_range(n, lo, hi);
+    // a range of at most kXq particles queues at most kXq crossings (one per
+    // particle): one-speed launches then cross them all after the block's last
+    // pass, and its waves need not meet after every pass to check the queue's
+    // fill (continuous-energy launches measured faster crossing as they go)
+    const bool one_flush = !kTables && hi - lo <= kXq;
     if (threadIdx.x == 0) { xn = 0; cn = 0; sn = 0; xf = 0; }
     __syncthreads();
     const uint32_t ident = A.ctr->qa_ident;
@@ -1673,15 +1678,26 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
             if (st == ST_SURF) A.qS[lo + xi] = s;
         } else {
             if (st == ST_SURF) xq[xi] = s;
-            __syncthreads();
-            const uint32_t q = xn;          // the same in every thread between the barriers
-            __syncthreads();
-            if (q + blockDim.x > kXq || base + blockDim.x >= hi) {
-                for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
+            if (!one_flush) {
                 __syncthreads();
-                if (threadIdx.x == 0) { sn += q; xn = 0; }
+                const uint32_t q = xn;      // the same in every thread between the barriers
                 __syncthreads();
+                if (q + blockDim.x > kXq || base + blockDim.x >= hi) {
+                    for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
+                    __syncthreads();
+                    if (threadIdx.x == 0) { sn += q; xn = 0; }
+                    __syncthreads();
+                }
             }
+        }
+    }
+    if constexpr (!kGrad) {
+        if (one_flush) {
+            // every crossing of the range is queued: the block crosses them once
+            __syncthreads();
+            const uint32_t q = xn;
+            for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
+            if (threadIdx.x == 0) sn += q;
         }
     }
     if constexpr (kGrad) {
