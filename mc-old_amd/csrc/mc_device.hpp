@@ -81,8 +81,10 @@ struct Counters {
     uint64_t flights;             // Ntrack
     uint32_t err_lost, err_split;
     uint64_t tA, tS, tC;          // particles processed by flight / surface / collision
-    uint32_t qa_ident, pad;       // 1: every slot below active_cap is live and queue A is
+    uint32_t qa_ident;            // 1: every slot below active_cap is live and queue A is
                                   // the identity (not written; kernels use the slot index)
+    uint32_t fcur;                // next queue-A entry a continuous-energy forward flight
+                                  // workgroup takes (reset by each refill)
     uint64_t chead, ctail;        // copy ring: split copies waiting for a slot are [chead, ctail)
     uint32_t cplaced, pad2;       // copies the last refill placed
     uint64_t xfl;                 // flights taken beyond the first of an iteration (included in tA)

@@ -850,6 +850,7 @@ __global__ void __launch_bounds__(1024) k_refill_scatter(KArgs A)
         A.ctr->cplaced = cmode ? (uint32_t)R.lim : 0u;
         A.ctr->nA = live_tot + (uint32_t)R.lim;
         A.ctr->qa_ident = R.ident ? 1u : 0u;
+        A.ctr->fcur = 0;
     }
     R.hb = next + (uint64_t)free_before;                  // the block's first new key
     // thread 0 finds the block's seed while the statuses load; the first
@@ -1611,14 +1612,31 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
     // pass, and its waves need not meet after every pass to check the queue's
     // fill (continuous-energy launches measured faster crossing as they go)
     const bool one_flush = !kTables && hi - lo <= kXq;
-    if (threadIdx.x == 0) { xn = 0; cn = 0; sn = 0; xf = 0; }
+    // continuous-energy forward launches: workgroups take blockDim-sized
+    // chunks of queue A from a cursor instead of fixed ranges, so that none
+    // holds the launch's end with a range of slow particles (a chunk's
+    // successor is claimed at its start and read between the pass's barriers)
+    constexpr bool kDyn = kTables && kFuse && !kGrad;
+    __shared__ uint32_t first, nxt;     // (the first chunk apart: nxt is rewritten at once)
+    if (threadIdx.x == 0) {
+        xn = 0; cn = 0; sn = 0; xf = 0;
+        if (kDyn) first = atomicAdd(&A.ctr->fcur, blockDim.x);
+    }
     __syncthreads();
+    if (kDyn) {
+        lo = first;
+        hi = n;
+    }
     const uint32_t ident = A.ctr->qa_ident;
     uint32_t s_next = lo + threadIdx.x < hi ? qa_at(A.qA, ident, lo + threadIdx.x) : 0u;   // read one item ahead
-    for (uint32_t base = lo; base < hi; base += blockDim.x) {
+    for (uint32_t base = lo; base < hi;) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t s = s_next;
-        if (i + blockDim.x < hi) s_next = qa_at(A.qA, ident, i + blockDim.x);
+        if (kDyn) {
+            if (threadIdx.x == 0) nxt = atomicAdd(&A.ctr->fcur, blockDim.x);
+        } else if (i + blockDim.x < hi) {
+            s_next = qa_at(A.qA, ident, i + blockDim.x);
+        }
         uint8_t st = ST_DEAD;
         if (i < hi) {
             bool crossed;
@@ -1681,15 +1699,22 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
             if (!one_flush) {
                 __syncthreads();
                 const uint32_t q = xn;      // the same in every thread between the barriers
+                const uint32_t next = kDyn ? nxt : base + blockDim.x;
                 __syncthreads();
-                if (q + blockDim.x > kXq || base + blockDim.x >= hi) {
+                if (q + blockDim.x > kXq || next >= hi) {
                     for (uint32_t j = threadIdx.x; j < q; j += blockDim.x) cross_deferred<kGrad>(A, xq[j]);
                     __syncthreads();
                     if (threadIdx.x == 0) { sn += q; xn = 0; }
                     __syncthreads();
                 }
+                if (kDyn) {
+                    base = next;
+                    s_next = base + threadIdx.x < hi ? qa_at(A.qA, ident, base + threadIdx.x) : 0u;
+                    continue;
+                }
             }
         }
+        base += blockDim.x;
     }
     if constexpr (!kGrad) {
         if (one_flush) {
