@@ -8,10 +8,14 @@ from mc_testlib import decks, mc, oracle_run, reaction_slab, shield_xs_dir
 pytestmark = pytest.mark.gpu
 
 
-def test_slab_backward_matches_oracle():
+# workgroups of 512 and 1024 threads: the surface-crossing kernel runs at 256
+# threads per workgroup and finds each flight workgroup's queue segment from
+# the flight's workgroup size
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_slab_backward_matches_oracle(block):
     n = 50000
     p = mc.Problem.from_string(decks.slab_deck(2.0, 0.6, 4, n, params=("scatter", "capture")))
-    eng = mc.Engine(p, bank_capacity=1 << 14)
+    eng = mc.Engine(p, bank_capacity=1 << 14, block=block)
     go = np.zeros(p.n_bins)
     go[0], go[1], go[2:] = 1.0, -0.5, 0.25
     g = eng.backward(go)
@@ -35,11 +39,11 @@ def test_reaction_score_backward_matches_oracle():
     np.testing.assert_allclose(g.sum, o.sum, rtol=1e-12)
 
 
-@pytest.mark.parametrize("drain", [1, -1])
-def test_shield_backward_matches_oracle(drain):
+@pytest.mark.parametrize("drain,block", [(1, 256), (-1, 256), (1, 512)])
+def test_shield_backward_matches_oracle(drain, block):
     n = 6000
     p = mc.Problem.from_string(decks.shield_deck(n, params="24"), shield_xs_dir())
-    eng = mc.Engine(p, bank_capacity=1 << 12, drain_divisor=drain)
+    eng = mc.Engine(p, bank_capacity=1 << 12, drain_divisor=drain, block=block)
     rng = np.random.default_rng(2)
     go = rng.normal(size=p.n_bins)
     theta = rng.uniform(0.9, 1.1, p.n_params)
