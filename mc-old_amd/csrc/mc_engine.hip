@@ -28,7 +28,7 @@ __global__ void k_count_events(const uint32_t* wg_cnt, uint32_t n_wg, Counters* 
 __global__ void k_xs_lookup(KArgs A, int use_lds);
 template <bool kTables, bool kGrad, bool kFuse, int kGeo> __global__ void k_flight(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_collision(KArgs A);
-template <bool kTables> __global__ void k_cross(KArgs A, uint32_t flight_bd);
+template <bool kTables> __global__ void k_cross(KArgs A);
 template <bool kTables, bool kGrad> __global__ void k_drain(KArgs A, uint32_t max_events);
 __global__ void k_drain_close(Counters* c);
 template <int kU> __global__ void k_xs_lookup_array(KArgs A, const double* E, const int32_t* mat, double* out, uint64_t n);
@@ -978,11 +978,8 @@ void Engine::Impl::launch_iteration(const KArgs& A, bool grad, hipStream_t st, h
     }
     // gradient runs cross surfaces in a kernel of their own (timed with the flight)
     if (grad) {
-        // (workgroup b takes flight workgroup b's segment: the same grid, at
-        // most 256 threads, so that the crossing code needs no scratch)
-        const dim3 bx(std::min(b.x, 256u));
-        if (has_tables) hipLaunchKernelGGL(k_cross<true>, g, bx, 0, st, A, (uint32_t)b.x);
-        else hipLaunchKernelGGL(k_cross<false>, g, bx, 0, st, A, (uint32_t)b.x);
+        if (has_tables) hipLaunchKernelGGL(k_cross<true>, g, b, 0, st, A);
+        else hipLaunchKernelGGL(k_cross<false>, g, b, 0, st, A);
     }
     mark(3, 1);
     mark(4, 0);

@@ -59,10 +59,9 @@ __device__ __forceinline__ void block_sum2(uint32_t& a, uint32_t& b, uint32_t* l
 }
 
 // Block-contiguous partition of n items over the grid, in whole blocks.
-__device__ __forceinline__ void block_range(uint32_t n, uint32_t& lo, uint32_t& hi, uint32_t bd = 0)
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t& lo, uint32_t& hi)
 {
-    const uint32_t nb = gridDim.x;
-    if (bd == 0) bd = blockDim.x;       // (else: the workgroup size of the launch whose ranges these are)
+    const uint32_t nb = gridDim.x, bd = blockDim.x;
     uint32_t per = (n + nb - 1) / nb;
     per = (per + bd - 1) / bd * bd;
     lo = blockIdx.x * per;
@@ -1746,15 +1745,12 @@ __global__ void __launch_bounds__(1024) k_flight(KArgs A)
 }
 
 // Gradient runs: the surface crossings k_flight queued, workgroup b taking
-// flight workgroup b's segment of queue S (the same grid; flight_bd: the
-// flight's workgroup size, which sets where segment b starts; this launch
-// runs at most 256 threads per workgroup, so that the crossing code needs no
-// scratch).
+// flight workgroup b's segment of queue S.
 template <bool kTables>
-__global__ void __launch_bounds__(256) k_cross(KArgs A, uint32_t flight_bd)
+__global__ void __launch_bounds__(1024) k_cross(KArgs A)
 {
     uint32_t lo, hi;
-    block_range(A.ctr->nA, lo, hi, flight_bd);
+    block_range(A.ctr->nA, lo, hi);
     const uint32_t n = A.wg_cnt[2 * blockIdx.x];
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) cross_deferred<true>(A, A.qS[lo + i]);
 }
@@ -2346,7 +2342,7 @@ template __global__ void k_flight<true, false, true, kGeoLds>(KArgs);
 template __global__ void k_flight<true, true, true, kGeoLds>(KArgs);
 template __global__ void k_flight<true, false, true, kGeoRec>(KArgs);
 template __global__ void k_flight<true, true, false, kGeoArrays>(KArgs);
-template __global__ void k_cross<false>(KArgs, uint32_t);
-template __global__ void k_cross<true>(KArgs, uint32_t);
+template __global__ void k_cross<false>(KArgs);
+template __global__ void k_cross<true>(KArgs);
 
 }  // namespace mc
