@@ -289,6 +289,27 @@ def kernel_roofline(mc, prob, H, traffic):
     return roof, table
 
 
+def bwd_fwd_leg(mc, decks, xs, H, use_graph):
+    """Likelihood-ratio backward cost against the forward's: the
+    24-parameter shield at H histories, one timed run of each after a warm-up
+    at H/10, VJP weights one over all 201 tally bins."""
+    p3 = mc.Problem.from_string(decks.shield_deck(H, params="24"), xs)
+    e3 = mc.Engine(p3, use_graph=use_graph)
+    n3 = H                                    # the differentiable-shield case runs 1e8 histories
+    e3.forward(n_histories=n3 // 10)
+    go = np.ones(e3.n_bins)          # VJP weights: the transmitted fraction and every leakage bin
+    e3.backward(go, n_histories=n3 // 10)
+    t = time.perf_counter()
+    e3.forward(n_histories=n3)
+    tf = time.perf_counter() - t
+    t = time.perf_counter()
+    b3 = e3.backward(go, n_histories=n3)
+    tbw = time.perf_counter() - t
+    return {"histories": n3, "forward_s": tf, "backward_s": tbw, "ratio": tbw / tf,
+            "params": int(e3.n_params), "grad_out": "ones over all 201 tally bins",
+            "dT_dtheta_max_rel_sigma": float(np.max(b3.grad_sigma / np.maximum(np.abs(b3.grad), 1e-300)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +325,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph replay")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--xs-only", action="store_true", help="only the standalone cross-section lookup")
+    ap.add_argument("--bwd-only", action="store_true",
+                    help="only the 24-parameter shield's forward and backward (for profiling the gradient run)")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the headline deck's per-kernel profile run and its roofline line "
                          "(for profiling the dominant kernel)")
@@ -317,11 +340,15 @@ def main():
               "(histories shard with no communication; see the sharding design)", file=sys.stderr)
         sys.exit(2)
 
-    if args.xs_only:
+    if args.xs_only or args.bwd_only:
         mc = importlib.import_module("mc-old_amd")
         decks = importlib.import_module("mc-old_amd.decks")
         xs = shield_xs_dir()
-        print(json.dumps(xs_lookup_bench(mc, decks, xs)), flush=True)
+        if args.xs_only:
+            print(json.dumps(xs_lookup_bench(mc, decks, xs)), flush=True)
+        else:
+            print(json.dumps({"bwd_fwd": bwd_fwd_leg(mc, decks, xs, int(args.histories), not args.no_graph)}),
+                  flush=True)
         return
     traffic = None
     if not (args.quick or args.no_traffic):
@@ -490,22 +517,9 @@ def main():
         log(f"sweep: batched {rate_b:.4g} hist/s ({tb:.2f} s), plain {rate_p:.4g} hist/s, x{rate_b / rate_p:.2f}")
 
         # ------------------------------------------------ backward cost
-        p3 = mc.Problem.from_string(decks.shield_deck(H, params="24"), xs)
-        e3 = mc.Engine(p3, use_graph=G)
-        n3 = H                                    # the differentiable-shield case runs 1e8 histories
-        e3.forward(n_histories=n3 // 10)
-        go = np.ones(e3.n_bins)          # VJP weights: the transmitted fraction and every leakage bin
-        e3.backward(go, n_histories=n3 // 10)
-        t = time.perf_counter()
-        f3 = e3.forward(n_histories=n3)
-        tf = time.perf_counter() - t
-        t = time.perf_counter()
-        b3 = e3.backward(go, n_histories=n3)
-        tbw = time.perf_counter() - t
-        out["bwd_fwd"] = {"histories": n3, "forward_s": tf, "backward_s": tbw, "ratio": tbw / tf,
-                          "params": int(e3.n_params), "grad_out": "ones over all 201 tally bins",
-                          "dT_dtheta_max_rel_sigma": float(np.max(b3.grad_sigma / np.maximum(np.abs(b3.grad), 1e-300)))}
-        log(f"bwd/fwd: {tbw:.3f}/{tf:.3f} = {tbw / tf:.2f}")
+        out["bwd_fwd"] = bwd_fwd_leg(mc, decks, xs, H, G)
+        bf = out["bwd_fwd"]
+        log(f"bwd/fwd: {bf['backward_s']:.3f}/{bf['forward_s']:.3f} = {bf['ratio']:.2f}")
 
         # ------------------------------------------------ 3D box: 64 design variants batched
         nv = 64
